@@ -1,0 +1,277 @@
+// Device building blocks shared by the fused MLP forward and backward kernels.
+#pragma once
+#include "common.h"
+#include "mlp_plan.h"
+
+// (W, D, skip, L_xyz, L_dir) topologies with compiled kernels:
+//   reference coarse/fine NeRF MLP, finite-difference check MLP, proposal MLP.
+#define NERF_MLP_INSTANCES(X) \
+  X(256, 8, 4, 10, 4)         \
+  X(32, 4, 1, 10, 4)          \
+  X(64, 4, -1, 8, 0)
+
+template <int W, int D, int SKIP, int LX, int LD>
+struct NetC {
+  static constexpr int NB = W / 32;        // output blocks of a hidden layer
+  static constexpr int KH = W / 2;         // fp32 k-steps over a hidden input
+  static constexpr int QH = KH / 4;        // groups of 4 k-steps
+  static constexpr int HX = 2 + 3 * LX;
+  static constexpr int KX = (HX + 7) / 8 * 8;
+  static constexpr bool RGB = LD > 0;
+  static constexpr int HD = RGB ? 2 + 3 * LD : 0;
+  static constexpr int KD = (HD + 7) / 8 * 8;
+  static constexpr int KDA = RGB ? KD : 8;  // array size (unused without rgb branch)
+  static constexpr int WR = W / 2;
+  static constexpr int NBR = (WR + 31) / 32;
+};
+
+int nerf_plan_from_desc(const nerf_mlp_desc* d, int prec, MlpPlan* p);
+int nerf_num_cus();
+int nerf_dispatch_fwd(const MlpPlan& P, const void* packed, int64_t M, const float* xyz,
+                      const float* dirs, const float* ro, const float* rd, const float* tv,
+                      int spr, float* rgb, float* sigma, void* stash, hipStream_t s);
+template <int W, int D, int SKIP, int LX, int LD>
+int launch_fwd_bf16(const MlpPlan& P, const void* packed, int64_t M, const float* xyz,
+                    const float* dirs, const float* ro, const float* rd, const float* tv, int spr,
+                    float* rgb, float* sigma, void* stash, hipStream_t s);
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float softplus(float z) { return z > 20.f ? z : log1pf(expf(z)); }
+__device__ __forceinline__ float sigmoidf(float z) { return 1.f / (1.f + expf(-z)); }
+
+// Sample point: either explicit (xyz, dirs) rows or o + t*d of its ray
+// (ray_utils.py:251, 389: one multiply, one add).
+__device__ __forceinline__ void load_sample(int64_t m, const float* __restrict__ xyz,
+                                            const float* __restrict__ dirs,
+                                            const float* __restrict__ ro,
+                                            const float* __restrict__ rd,
+                                            const float* __restrict__ tv, int spr, float (&x)[3],
+                                            float (&d)[3]) {
+  if (ro) {
+    const int64_t r = m / spr;
+    const float t = tv[m];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      d[c] = rd[3 * r + c];
+      x[c] = fadd(ro[3 * r + c], fmul(t, d[c]));
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[c] = xyz[3 * m + c];
+      d[c] = dirs ? dirs[3 * m + c] : 0.f;
+    }
+  }
+}
+
+// Positional encoding (model.py:309-330) directly in the lane-half slots of
+// mlp_plan.h.  Argument = fl32(x_c * (2^l * fl32(pi))), exact power-of-two
+// scaling of fl32(pi) as in the reference; sin/cos are full-range accurate.
+template <int L, int K>
+__device__ __forceinline__ void encode_slots(const float (&x)[3], int g, float (&out)[K]) {
+  constexpr int P0 = (3 * L - 2) / 2;
+  constexpr float kPi = 3.14159265358979323846f;
+#pragma unroll
+  for (int k = 0; k < K / 2; ++k) {
+    const int p = g ? P0 + k : k - 2;
+    const bool is_pair = g ? (P0 + k < 3 * L) : (k >= 2 && k - 2 < P0);
+    float s = 0.f, c = 0.f;
+    if (is_pair) {
+      const int comp = p / L, l = p - comp * L;
+      const float xc = comp == 0 ? x[0] : (comp == 1 ? x[1] : x[2]);
+      const float arg = fmul(xc, ldexpf(kPi, l));
+      sincosf(arg, &s, &c);
+    }
+    float v0 = s, v1 = c;
+    if (!is_pair) {
+      v0 = (g == 0 && k == 0) ? x[0] : ((g == 0 && k == 1) ? x[2] : 0.f);
+      v1 = (g == 0 && k == 0) ? x[1] : 0.f;
+    }
+    out[2 * k] = v0;
+    out[2 * k + 1] = v1;
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void store_slots_f32(const float (&v)[K], float* dst) {
+#pragma unroll
+  for (int e = 0; e < K; e += 4)
+    *reinterpret_cast<floatx4*>(dst + e) = floatx4{v[e], v[e + 1], v[e + 2], v[e + 3]};
+}
+
+template <int NBo>
+__device__ __forceinline__ void zero_acc(floatx16 (&acc)[NBo]) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[nb][i] = 0.f;
+}
+
+__device__ __forceinline__ floatx16 mfma_f32(float a, float b, floatx16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// acc[nb] += A(q-group, nb) * B for KR register k-steps (KR % 4 == 0).  Two
+// weight groups in flight; scheduling barriers keep the compiler from hoisting
+// every load of the unrolled loop (which would need KR*NBo VGPRs).
+template <int NBo, int KR>
+__device__ __forceinline__ void mfma_regs_f32(floatx16 (&acc)[NBo], const floatx4* __restrict__ wq,
+                                              const float (&b)[KR], int lane) {
+  constexpr int G = KR / 4;
+  floatx4 wa[NBo], wb[NBo];
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[nb * 64 + lane];
+#pragma unroll
+  for (int q = 0; q < G; q += 2) {
+    if (q + 1 < G) {
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) wb[nb] = wq[((q + 1) * NBo + nb) * 64 + lane];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wa[nb][r], b[4 * q + r], acc[nb]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (q + 2 < G) {
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[((q + 2) * NBo + nb) * 64 + lane];
+    }
+    if (q + 1 < G) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wb[nb][r], b[4 * (q + 1) + r], acc[nb]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// acc[nb] += A * B over Q groups of 4 k-steps whose B operand is in the wave's
+// LDS slab act[q][lane]; weights are prefetched one group ahead.
+template <int NBo, int Q>
+__device__ __forceinline__ void mfma_lds_f32(floatx16 (&acc)[NBo], const floatx4* __restrict__ wq,
+                                             const floatx4* act, int lane) {
+  static_assert(Q % 2 == 0, "Q must be even");
+  floatx4 wa[NBo], wb[NBo];
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[nb * 64 + lane];
+  for (int q = 0; q < Q; q += 2) {
+#pragma unroll
+    for (int nb = 0; nb < NBo; ++nb) wb[nb] = wq[((q + 1) * NBo + nb) * 64 + lane];
+    floatx4 b = act[q * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wa[nb][r], b[r], acc[nb]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (q + 2 < Q) {
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[((q + 2) * NBo + nb) * 64 + lane];
+    }
+    b = act[(q + 1) * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wb[nb][r], b[r], acc[nb]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// z = acc + bias, then ReLU (RELU = 1) or identity.
+template <int NBo, int RELU>
+__device__ __forceinline__ void bias_act(floatx16 (&acc)[NBo], const float* __restrict__ bias, int g) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const floatx4 bv = *reinterpret_cast<const floatx4*>(bias + 32 * nb + 8 * a + 4 * g);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v = acc[nb][4 * a + r] + bv[r];
+        if (RELU) v = v > 0.f ? v : 0.f;
+        acc[nb][4 * a + r] = v;
+      }
+    }
+}
+
+// Accumulators -> B-operand slab: k-step 16 nb + i lives at act[(16nb+i)/4][lane][i%4].
+template <int NBo>
+__device__ __forceinline__ void store_act_f32(const floatx16 (&acc)[NBo], floatx4* act, int lane) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      act[(4 * nb + a) * 64 + lane] =
+          floatx4{acc[nb][4 * a], acc[nb][4 * a + 1], acc[nb][4 * a + 2], acc[nb][4 * a + 3]};
+}
+
+// Accumulators -> a row-major feature row of this lane's sample (natural order).
+template <int NBo>
+__device__ __forceinline__ void store_rows_f32(const floatx16 (&acc)[NBo], float* row, int g) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      *reinterpret_cast<floatx4*>(row + 32 * nb + 8 * a + 4 * g) =
+          floatx4{acc[nb][4 * a], acc[nb][4 * a + 1], acc[nb][4 * a + 2], acc[nb][4 * a + 3]};
+}
+
+// Row-major feature row -> accumulator layout.
+template <int NBo>
+__device__ __forceinline__ void load_rows_f32(floatx16 (&acc)[NBo], const float* row, int g) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const floatx4 v = *reinterpret_cast<const floatx4*>(row + 32 * nb + 8 * a + 4 * g);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[nb][4 * a + r] = v[r];
+    }
+}
+
+// Partial dot product of this lane's features with a head weight vector laid
+// out by k-step (w[16 nb + i] for feature cfeat(nb, i, g)).
+template <int NBo>
+__device__ __forceinline__ float head_dot_f32(const floatx16 (&acc)[NBo], const float* __restrict__ w) {
+  float s = 0.f;
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const floatx4 wv = *reinterpret_cast<const floatx4*>(w + 16 * nb + 4 * a);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s = s + acc[nb][4 * a + r] * wv[r];
+    }
+  return s;
+}
+
+// rgb = sigmoid(h @ W_rgb + b) with W_rgb packed as [g][k-step][4].
+template <int NBR>
+__device__ __forceinline__ void rgb_head_f32(const floatx16 (&accr)[NBR], const float* __restrict__ hr,
+                                             int g, float (&rgb)[3]) {
+  constexpr int KS = 16 * NBR;
+  float p0 = 0.f, p1 = 0.f, p2 = 0.f;
+  const floatx4* w4 = reinterpret_cast<const floatx4*>(hr) + g * KS;
+#pragma unroll
+  for (int nb = 0; nb < NBR; ++nb)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const floatx4 w = w4[16 * nb + i];
+      const float h = accr[nb][i];
+      p0 = p0 + h * w[0];
+      p1 = p1 + h * w[1];
+      p2 = p2 + h * w[2];
+    }
+  p0 = p0 + shfl_xor(p0, 32);
+  p1 = p1 + shfl_xor(p1, 32);
+  p2 = p2 + shfl_xor(p2, 32);
+  const float* bias = hr + 2 * KS * 4;
+  rgb[0] = sigmoidf(p0 + bias[0]);
+  rgb[1] = sigmoidf(p1 + bias[1]);
+  rgb[2] = sigmoidf(p2 + bias[2]);
+}

@@ -1,0 +1,208 @@
+// Layout plan of the fused NeRF MLP: the flat "Keras" parameter layout, the
+// packed MFMA operand layout, the training stash and the per-sample gradients.
+//
+// Register layout (v_mfma_f32_32x32x{2f32,16_bf16}): one wave owns 32 samples.
+// Lane l = j + 32 g (j = sample in the block, g = lane half).  An MFMA output
+// block nb (32 features) holds in accumulator i of lane l the feature
+//   F(nb, i, g) = 32 nb + 8 (i/4) + 4 g + (i%4)
+// of sample j.  Those accumulators are fed back as the B operand of the next
+// layer without any transpose: fp32 k-step s = 16 nb + i uses F(nb, i, g) on
+// lane half g; bf16 k-step t uses slots r = 0..7 of F(t/2, 8 (t%2) + r, g).
+// The packed A operand (weights) is permuted to match, so the k order of each
+// dot product is the only thing that differs from a plain matmul.
+//
+// Positional-encoding features are produced directly in lane slots: half g
+// holds slots e = 0..H-1 (H = 2 + 3L).  Half 0: x, y, z, pad, then (sin, cos)
+// pairs 0..P0-1; half 1: pairs P0..3L-1 (P0 = (3L-2)/2).  Pair p encodes
+// component c = p / L at frequency l = p % L, i.e. reference features
+// 3 + c 2L + 2l + {0: sin, 1: cos} (model.py:323-330).
+#pragma once
+#include <stdint.h>
+
+#define NERF_MAX_LAYERS 16
+
+__host__ __device__ inline int ru(int x, int m) { return (x + m - 1) / m * m; }
+
+// Reference feature index of encoding slot e on lane half g, or -1 for a pad slot.
+__host__ __device__ inline int enc_feat(int g, int e, int L) {
+  const int H = 2 + 3 * L, P0 = (3 * L - 2) / 2;
+  if (e < 0 || e >= H) return -1;
+  int p, sc;
+  if (g == 0) {
+    if (e < 3) return e;
+    if (e == 3) return -1;
+    p = (e - 4) / 2;
+    sc = (e - 4) % 2;
+  } else {
+    p = P0 + e / 2;
+    sc = e % 2;
+  }
+  const int c = p / L, l = p % L;
+  return 3 + c * 2 * L + 2 * l + sc;
+}
+
+// Hidden feature held by accumulator i of output block nb on lane half g.
+__host__ __device__ inline int cfeat(int nb, int i, int g) { return 32 * nb + 8 * (i / 4) + 4 * g + (i % 4); }
+
+struct MlpPlan {
+  int D, W, skip, LX, LD, rgb;
+  int NB, NBR, WR;      // hidden blocks, rgb-branch blocks, rgb-branch width
+  int HX, HD;           // encoding slots per lane half
+  int KX, KD;           // encoding slots per half padded to 8 (= fp32 k-steps)
+  int KX16, KD16;       // bf16 k-steps for the encodings
+  int FX, FD;           // encoding widths (3 + 6L)
+  int n_keras;          // Keras layers in trainable-variable order
+  int fan_in[NERF_MAX_LAYERS], fan_out[NERF_MAX_LAYERS];
+  int64_t k_off[NERF_MAX_LAYERS], b_off[NERF_MAX_LAYERS];
+  int64_t n_params;
+  // MFMA layers: m = 0..D-1 trunk, D bottleneck, D+1 rgb branch (index into arrays below)
+  int n_mfma;
+  int keras_of[NERF_MAX_LAYERS];   // Keras layer index of MFMA layer m
+  int ks_hid[NERF_MAX_LAYERS];     // fp32 k-steps read from the hidden input (0 for m = 0)
+  int ks_enc[NERF_MAX_LAYERS];     // fp32 k-steps from an encoding (KX / KD / 0)
+  int enc_L[NERF_MAX_LAYERS];      // L of that encoding
+  int nb_out[NERF_MAX_LAYERS];     // output blocks
+  int relu[NERF_MAX_LAYERS];
+  // packed byte offsets
+  int64_t p_fwd[NERF_MAX_LAYERS], p_bwd[NERF_MAX_LAYERS], p_bias[NERF_MAX_LAYERS];
+  int64_t p_head_sigma, p_head_rgb, packed_bytes;
+  int prec;
+  // stash (elements per sample; fp32 or bf16 elements) and dz (floats per sample)
+  int64_t st_encx, st_h[NERF_MAX_LAYERS], st_bott, st_rbr, st_encd, stash_per_sample;
+  int64_t dz_h[NERF_MAX_LAYERS], dz_bott, dz_rbr, dz_sigma, dz_rgb, dz_per_sample;
+};
+
+// Returns 0 on success, -1 if the description is not supported.
+inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPlan* p) {
+  // skip < D-1: the sigma head and the bottleneck read the last trunk layer only
+  if (D < 2 || D > 10 || W < 32 || W > 256 || (W % 32) || skip < -1 || skip >= D - 1 || LX < 2 ||
+      (LX % 2) || LX > 16 || LD < 0 || (LD % 2) || LD > 16 || (prec != 0 && prec != 1))
+    return -1;
+  MlpPlan& P = *p;
+  P = MlpPlan{};
+  P.D = D; P.W = W; P.skip = skip; P.LX = LX; P.LD = LD; P.rgb = LD > 0; P.prec = prec;
+  P.NB = W / 32;
+  P.WR = W / 2;
+  P.NBR = (P.WR + 31) / 32;
+  P.HX = 2 + 3 * LX;
+  P.HD = LD ? 2 + 3 * LD : 0;
+  P.KX = ru(P.HX, 8);
+  P.KD = LD ? ru(P.HD, 8) : 0;
+  P.KX16 = P.KX / 8;
+  P.KD16 = P.KD / 8;
+  P.FX = 3 + 6 * LX;
+  P.FD = LD ? 3 + 6 * LD : 0;
+  // Keras layers
+  int n = 0;
+  const int trunk_out_in = W + (skip == D - 1 ? P.FX : 0);
+  for (int i = 0; i < D; ++i) {
+    P.fan_in[n] = i == 0 ? P.FX : W + (i - 1 == skip ? P.FX : 0);
+    P.fan_out[n] = W;
+    ++n;
+  }
+  P.fan_in[n] = trunk_out_in; P.fan_out[n] = 1; ++n;  // sigma
+  if (P.rgb) {
+    P.fan_in[n] = trunk_out_in; P.fan_out[n] = W; ++n;        // dense_D (bottleneck)
+    P.fan_in[n] = W + P.FD; P.fan_out[n] = P.WR; ++n;         // dense_{D+1}
+    P.fan_in[n] = P.WR; P.fan_out[n] = 3; ++n;                // rgb
+  }
+  P.n_keras = n;
+  int64_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    P.k_off[i] = off;
+    off += (int64_t)P.fan_in[i] * P.fan_out[i];
+    P.b_off[i] = off;
+    off += P.fan_out[i];
+  }
+  P.n_params = off;
+  // MFMA layers
+  int m = 0;
+  for (int i = 0; i < D; ++i) {
+    P.keras_of[m] = i;
+    P.ks_hid[m] = i == 0 ? 0 : W / 2;
+    P.ks_enc[m] = i == 0 ? P.KX : (i - 1 == skip ? P.KX : 0);
+    P.enc_L[m] = LX;
+    P.nb_out[m] = P.NB;
+    P.relu[m] = 1;
+    ++m;
+  }
+  if (P.rgb) {
+    P.keras_of[m] = D + 1;
+    P.ks_hid[m] = W / 2;
+    P.ks_enc[m] = skip == D - 1 ? P.KX : 0;
+    P.enc_L[m] = LX;
+    P.nb_out[m] = P.NB;
+    P.relu[m] = 0;
+    ++m;
+    P.keras_of[m] = D + 2;
+    P.ks_hid[m] = W / 2;
+    P.ks_enc[m] = P.KD;
+    P.enc_L[m] = LD;
+    P.nb_out[m] = P.NBR;
+    P.relu[m] = 1;
+    ++m;
+  }
+  P.n_mfma = m;
+  // packed layout
+  int64_t b = 0;
+  auto align = [&](int64_t a) { b = (b + a - 1) / a * a; };
+  for (int i = 0; i < P.n_mfma; ++i) {
+    align(256);
+    P.p_fwd[i] = b;
+    if (prec == 0)
+      b += (int64_t)(P.ks_hid[i] + P.ks_enc[i]) * P.nb_out[i] * 64 * 4;
+    else {
+      const int ke16 = P.ks_enc[i] == 0 ? 0 : (P.enc_L[i] == LX ? P.KX16 : P.KD16);
+      b += (int64_t)(P.ks_hid[i] / 8 + ke16) * P.nb_out[i] * 64 * 16;
+    }
+  }
+  for (int i = 0; i < P.n_mfma; ++i) {
+    P.p_bwd[i] = -1;
+    if (i == 0) continue;  // no dX for the encoding input
+    align(256);
+    P.p_bwd[i] = b;
+    // A[n = input hidden feature][k = output feature]: k-steps over the outputs
+    const int out_ks = P.nb_out[i] * 16;  // fp32 k-steps over 32*nb_out outputs
+    if (prec == 0)
+      b += (int64_t)out_ks * P.NB * 64 * 4;
+    else
+      b += (int64_t)(out_ks / 8) * P.NB * 64 * 16;
+  }
+  for (int i = 0; i < P.n_mfma; ++i) {
+    align(256);
+    P.p_bias[i] = b;
+    b += (int64_t)P.nb_out[i] * 32 * 4;
+  }
+  align(256);
+  P.p_head_sigma = b;  // [2][W/2] fp32 + bias (padded to 4)
+  b += (int64_t)2 * (W / 2) * 4 + 16;
+  align(256);
+  P.p_head_rgb = b;  // [2][16*NBR][4] fp32 + bias[4]
+  if (P.rgb) b += (int64_t)2 * 16 * P.NBR * 16 + 16;
+  align(256);
+  P.packed_bytes = b;
+  // stash (elements per sample)
+  int64_t s = 0;
+  P.st_encx = s; s += 2 * P.KX;
+  for (int i = 0; i < D; ++i) { P.st_h[i] = s; s += W; }
+  P.st_bott = P.st_rbr = P.st_encd = -1;
+  if (P.rgb) {
+    P.st_bott = s; s += W;
+    P.st_rbr = s; s += 32 * P.NBR;
+    P.st_encd = s; s += 2 * P.KD;
+  }
+  P.stash_per_sample = s;
+  // dz (floats per sample)
+  int64_t z = 0;
+  for (int i = 0; i < D; ++i) { P.dz_h[i] = z; z += W; }
+  P.dz_bott = P.dz_rbr = P.dz_rgb = -1;
+  if (P.rgb) {
+    P.dz_bott = z; z += W;
+    P.dz_rbr = z; z += 32 * P.NBR;
+  }
+  P.dz_sigma = z; z += 1;
+  if (P.rgb) { P.dz_rgb = z; z += 3; }
+  z = ru((int)z, 4);
+  P.dz_per_sample = z;
+  return 0;
+}

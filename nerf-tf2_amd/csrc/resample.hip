@@ -1,0 +1,169 @@
+// Inverse-transform fine sampling: create_input_batch_fine_model
+// (nerf/utils/ray_utils.py:276-406).  One wave64 per ray.
+//
+// Bit-exact with the oracle: every float op is an explicitly rounded
+// intrinsic and the summation orders are fixed —
+//   denom = reduce_sum(w' * width)  in the order of an 8-wide Eigen packet
+//           reduction (lane j sums bins j, j+8, ...; then
+//           ((p0+p4)+(p2+p6)) + ((p1+p5)+(p3+p7)), plus the n mod 8 tail);
+//   agg   = cumsum(pdf * width)      sequential, each lane recomputing its own
+//           prefix in left-to-right order.
+// searchsorted(side='right') is an upper_bound binary search; the final
+// tf.sort(concat(t_coarse, t_fine)) is a rank merge (coarse values first on
+// ties), so equal values land in equal slots and the output is the sorted set.
+#include "common.h"
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kMaxNc = 64;
+constexpr int kMaxNf = 256;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(256) void fine_kernel(
+    int64_t n_rays, int nc, int nf, const float* __restrict__ ro, const float* __restrict__ rd,
+    const float* __restrict__ bin_weights, const float* __restrict__ bin_edges,
+    const float* __restrict__ bin_widths, const float* __restrict__ t_c,
+    const float* __restrict__ u_vals, float* __restrict__ t_out, int32_t* __restrict__ idx_out,
+    float* __restrict__ tf_out, float* __restrict__ xyz, float* __restrict__ dirs) {
+  __shared__ float s_agg[kWaves][kMaxNc + 1];
+  __shared__ float s_pdf[kWaves][kMaxNc];
+  __shared__ float s_left[kWaves][kMaxNc];
+  __shared__ float s_tc[kWaves][kMaxNc];
+  __shared__ float s_tf[kWaves][kMaxNf];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t ray = (int64_t)blockIdx.x * kWaves + w;
+  if (ray >= n_rays) return;  // wave-uniform; no block-level barriers below
+  const int S = nc + nf;
+
+  // ---- pdf and cdf over the coarse bins (ray_utils.py:335-345)
+  const bool on = lane < nc;
+  const float wv = on ? bin_weights[ray * nc + lane] : 0.f;
+  const float width = on ? bin_widths[ray * nc + lane] : 0.f;
+  const float left = on ? bin_edges[ray * (nc + 1) + lane] : 0.f;
+  const float wp = fadd(wv, 1e-5f);      // bin_weights + 1e-5
+  const float prod = fmul(wp, width);    // bin_weights * bin_widths
+  const int nv = (nc / 8) * 8;
+  float p = 0.f;
+  for (int i = (lane & 7); i < nv; i += 8) p = fadd(p, shfl(prod, i));
+  // every lane needs the same shuffles: loop bound is lane-independent
+  float tail = 0.f;
+  for (int i = nv; i < nc; ++i) tail = fadd(tail, shfl(prod, i));
+  const float P0 = shfl(p, 0), P1 = shfl(p, 1), P2 = shfl(p, 2), P3 = shfl(p, 3);
+  const float P4 = shfl(p, 4), P5 = shfl(p, 5), P6 = shfl(p, 6), P7 = shfl(p, 7);
+  const float denom =
+      fadd(tail, fadd(fadd(fadd(P0, P4), fadd(P2, P6)), fadd(fadd(P1, P5), fadd(P3, P7))));
+  const float pdf = fdiv(wp, denom);     // ray_utils.py:339
+  const float cw = fmul(pdf, width);     // pdf * bin_widths
+  float agg = 0.f;
+  for (int k = 0; k < nc; ++k) {         // sequential inclusive cumsum (ray_utils.py:342)
+    const float v = shfl(cw, k);
+    if (k <= lane) agg = fadd(agg, v);
+  }
+  if (lane == 0) s_agg[w][0] = 0.f;      // concat 0 (ray_utils.py:345)
+  if (on) {
+    s_agg[w][lane + 1] = agg;
+    s_pdf[w][lane] = pdf;
+    s_left[w][lane] = left;
+    s_tc[w][lane] = t_c[ray * nc + lane];
+  }
+  wave_sync();
+
+  // ---- inverse transform (ray_utils.py:355-377)
+  for (int q = lane; q < nf; q += 64) {
+    const float u = u_vals[ray * nf + q];
+    // searchsorted(agg[:, 1:-1], u, side='right'): first inner edge > u
+    int lo = 0, hi = nc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_agg[w][1 + mid] <= u)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const int idx = lo;
+    const float agg_ = s_agg[w][idx];
+    float pdf_ = s_pdf[w][idx];
+    const float left_ = s_left[w][idx];
+    const float mask = pdf_ < 1e-8f ? 0.f : 1.f;   // ray_utils.py:372
+    pdf_ = fmaxf(pdf_, 1e-8f);                      // ray_utils.py:373
+    const float tf = fadd(fmul(fdiv(fsub(u, agg_), pdf_), mask), left_);  // :376
+    s_tf[w][q] = tf;
+    if (idx_out) idx_out[ray * nf + q] = idx;
+    if (tf_out) tf_out[ray * nf + q] = tf;
+  }
+  wave_sync();
+
+  // ---- sort(concat(t_coarse, t_fine)) as a rank merge (ray_utils.py:380-385)
+  float o3[3] = {0.f, 0.f, 0.f}, d3[3] = {0.f, 0.f, 0.f};
+  if (xyz) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      o3[c] = ro[3 * ray + c];
+      d3[c] = rd[3 * ray + c];
+    }
+  }
+  float* trow = t_out + ray * S;
+  auto emit = [&](int pos, float t) {
+    trow[pos] = t;
+    if (xyz) {  // o + t * d                                  (ray_utils.py:389-398)
+      const int64_t m = ray * S + pos;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xyz[3 * m + c] = fadd(o3[c], fmul(t, d3[c]));
+        dirs[3 * m + c] = d3[c];
+      }
+    }
+  };
+  for (int q = lane; q < nf; q += 64) {
+    const float v = s_tf[w][q];
+    int rank = 0;
+    for (int k = 0; k < nf; ++k) {
+      const float o = s_tf[w][k];
+      rank += (o < v) || (o == v && k < q);
+    }
+    int nle = 0;  // coarse values <= v (coarse first on ties)
+    for (int k = 0; k < nc; ++k) nle += (s_tc[w][k] <= v);
+    emit(rank + nle, v);
+  }
+  for (int j = lane; j < nc; j += 64) {
+    const float v = s_tc[w][j];
+    int nlt = 0;
+    for (int k = 0; k < nf; ++k) nlt += (s_tf[w][k] < v);
+    emit(j + nlt, v);
+  }
+}
+
+}  // namespace
+
+extern "C" int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse, int32_t n_fine,
+                                                const float* rays_o, const float* rays_d,
+                                                const float* bin_weights, const float* bin_edges,
+                                                const float* bin_widths,
+                                                const float* t_vals_coarse, const float* u_vals,
+                                                float* t_vals, int32_t* piece_idxs,
+                                                float* t_vals_fine, float* xyz_inputs,
+                                                float* dir_inputs, void* stream) {
+  NERF_CHECK_ARG(n_rays >= 0 && n_coarse >= 2 && n_coarse <= kMaxNc && n_fine >= 1 &&
+                     n_fine <= kMaxNf,
+                 "nerf_create_input_batch_fine: n_coarse=%d (2..%d) n_fine=%d (1..%d)", n_coarse,
+                 kMaxNc, n_fine, kMaxNf);
+  NERF_CHECK_ARG(bin_weights && bin_edges && bin_widths && t_vals_coarse && u_vals && t_vals,
+                 "nerf_create_input_batch_fine: null pointer");
+  NERF_CHECK_ARG((xyz_inputs == nullptr) == (dir_inputs == nullptr),
+                 "nerf_create_input_batch_fine: xyz_inputs and dir_inputs go together");
+  NERF_CHECK_ARG(!xyz_inputs || (rays_o && rays_d),
+                 "nerf_create_input_batch_fine: xyz_inputs needs rays");
+  if (n_rays == 0) return NERF_OK;
+  fine_kernel<<<(unsigned)((n_rays + kWaves - 1) / kWaves), 64 * kWaves, 0, as_stream(stream)>>>(
+      n_rays, n_coarse, n_fine, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse,
+      u_vals, t_vals, piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
+  NERF_LAUNCH_CHECK("nerf_create_input_batch_fine");
+  return NERF_OK;
+}

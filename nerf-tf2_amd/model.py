@@ -1,0 +1,316 @@
+"""The reference's model interface (nerf/core/model.py) on MI355X.
+
+* ``get_coarse_or_fine_model(model_name, num_units=256)`` (model.py:334-394)
+  returns a :class:`NerfMLP`: the encoder + 8x256 MLP with the layer-5 skip,
+  held as one flat float32 parameter vector in Keras trainable-variable order
+  and a packed copy in the MFMA operand layout (fp32 or bf16).  Calling it as
+  ``model((xyz, dirs))`` returns ``(rgb, sigma)`` like the Keras model.
+* :class:`NeRF` mirrors ``NeRF.forward`` / ``train_step`` / ``predict_step``
+  (model.py:57-237) with the Keras Adam + ExponentialDecay of model.py:413-418.
+
+All compute is in libnerf_hip.so; torch provides device memory, streams and
+the autograd tape.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import ray_utils
+from ._lib import call, ptr, stream
+
+Tensor = torch.Tensor
+
+
+@dataclass(frozen=True)
+class MLPConfig:
+    """Topology of get_coarse_or_fine_model (reference: depth 8, width 256, skip 4, L 10/4).
+    ``l_dir = 0`` makes a density-only net (the proposal network)."""
+    depth: int = 8
+    width: int = 256
+    skip: int = 4
+    l_xyz: int = 10
+    l_dir: int = 4
+
+    def desc(self) -> _lib.NerfMlpDesc:
+        return _lib.desc(self.depth, self.width, self.skip, self.l_xyz, self.l_dir)
+
+    def layer_shapes(self) -> List[Tuple[str, int, int]]:
+        """(name, fan_in, fan_out) in Keras trainable-variable order."""
+        fx = 3 + 6 * self.l_xyz
+        fd = 3 + 6 * self.l_dir
+        out, fan = [], fx
+        for i in range(self.depth):
+            out.append((f"dense_{i}", fan, self.width))
+            fan = self.width + (fx if i == self.skip else 0)
+        out.append(("sigma", fan, 1))
+        if self.l_dir:
+            out.append((f"dense_{self.depth}", fan, self.width))
+            out.append((f"dense_{self.depth + 1}", self.width + fd, self.width // 2))
+            out.append(("rgb", self.width // 2, 3))
+        return out
+
+    def macs_per_sample(self) -> int:
+        return sum(i * o for _, i, o in self.layer_shapes())
+
+
+PRECISIONS = {"fp32": _lib.PREC_F32, "bf16": _lib.PREC_BF16}
+
+
+class MLPFunction(torch.autograd.Function):
+    """Fused encoder+MLP forward/backward (HIP) as an autograd Function.
+    Inputs: flat params (differentiable), then the sample description."""
+
+    @staticmethod
+    def forward(ctx, params: Tensor, mlp: "NerfMLP", xyz, dirs, rays_o, rays_d, t_vals):
+        need_grad = ctx.needs_input_grad[0]
+        M = mlp._num_samples(xyz, t_vals)
+        dev = params.device
+        stash = mlp._alloc_stash(M) if need_grad else None
+        rgb = torch.empty((M, 3), dtype=torch.float32, device=dev) if mlp.config.l_dir else None
+        sigma = torch.empty((M, 1), dtype=torch.float32, device=dev)
+        mlp._launch_forward(M, xyz, dirs, rays_o, rays_d, t_vals, rgb, sigma, stash)
+        ctx.mlp = mlp
+        ctx.M = M
+        if need_grad:
+            ctx.save_for_backward(rgb if rgb is not None else sigma, sigma, stash)
+        if rgb is None:
+            return sigma
+        return rgb, sigma
+
+    @staticmethod
+    def backward(ctx, *grads):
+        mlp: NerfMLP = ctx.mlp
+        if mlp.config.l_dir:
+            g_rgb, g_sigma = grads
+        else:
+            g_rgb, g_sigma = None, grads[0]
+        rgb, sigma, stash = ctx.saved_tensors
+        if not mlp.config.l_dir:
+            rgb = None
+        grad = mlp.backward_to_params(ctx.M, g_rgb, g_sigma, rgb, sigma, stash)
+        return grad, None, None, None, None, None, None
+
+
+class NerfMLP:
+    """One coarse / fine (or proposal) network."""
+
+    def __init__(self, model_name: str = "coarse", config: MLPConfig = MLPConfig(), precision: str = "fp32",
+                 seed: int = 0, device="cuda"):
+        self.name = model_name
+        self.config = config
+        self.precision = precision
+        self.prec = PRECISIONS[precision]
+        self.device = torch.device(device)
+        self._desc = config.desc()
+        lib = _lib.load()
+        self.n_params = int(lib.nerf_mlp_param_count(ctypes_ref(self._desc)))
+        if self.n_params <= 0:
+            raise ValueError(f"unsupported MLP config {config}")
+        self.params = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+        self.glorot_init(seed)
+        self.packed = torch.empty(int(lib.nerf_mlp_packed_bytes(ctypes_ref(self._desc), self.prec)),
+                                  dtype=torch.uint8, device=self.device)
+        self.stash_per_sample = int(lib.nerf_mlp_stash_floats_per_sample(ctypes_ref(self._desc)))
+        self.dz_per_sample = int(lib.nerf_mlp_dz_floats_per_sample(ctypes_ref(self._desc)))
+        self.repack()
+
+    # -- parameters --------------------------------------------------------
+    def layer_slices(self):
+        """[(name, kernel view (fan_in, fan_out), bias view (fan_out,))] in Keras order."""
+        out, off = [], 0
+        for name, fi, fo in self.config.layer_shapes():
+            k = self.params[off:off + fi * fo].view(fi, fo)
+            off += fi * fo
+            b = self.params[off:off + fo]
+            off += fo
+            out.append((f"{self.name}/{name}", k, b))
+        return out
+
+    def glorot_init(self, seed: int) -> None:
+        """Keras defaults (Glorot-uniform kernels, zero biases) from a seeded numpy generator."""
+        rng = np.random.default_rng(seed)
+        flat = []
+        for _, fi, fo in self.config.layer_shapes():
+            lim = math.sqrt(6.0 / (fi + fo))
+            flat.append(rng.uniform(-lim, lim, size=(fi * fo,)).astype(np.float32))
+            flat.append(np.zeros((fo,), dtype=np.float32))
+        self.params.copy_(torch.from_numpy(np.concatenate(flat)))
+
+    def get_weights(self) -> List[np.ndarray]:
+        """Keras-style list [kernel_0, bias_0, ...] as numpy arrays."""
+        out = []
+        for _, k, b in self.layer_slices():
+            out += [k.detach().cpu().numpy(), b.detach().cpu().numpy()]
+        return out
+
+    def set_weights(self, weights: Sequence) -> None:
+        flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in weights])
+        if flat.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} values, got {flat.size}")
+        self.params.copy_(torch.from_numpy(flat))
+        self.repack()
+
+    def repack(self) -> None:
+        call("nerf_mlp_pack", ctypes_ref(self._desc), self.prec, ptr(self.params), ptr(self.packed),
+             stream(self.device))
+
+    # -- forward -----------------------------------------------------------
+    def _num_samples(self, xyz, t_vals) -> int:
+        return int(t_vals.numel()) if t_vals is not None else int(xyz.shape[0])
+
+    def _alloc_stash(self, M: int) -> Tensor:
+        if self.prec == _lib.PREC_F32:
+            return torch.empty((M, self.stash_per_sample), dtype=torch.float32, device=self.device)
+        return torch.empty((M, self.stash_per_sample), dtype=torch.bfloat16, device=self.device)
+
+    def _launch_forward(self, M, xyz, dirs, rays_o, rays_d, t_vals, rgb, sigma, stash):
+        c = lambda t: None if t is None else t.contiguous()
+        spr = int(t_vals.shape[1]) if t_vals is not None else 0
+        call("nerf_mlp_forward", ctypes_ref(self._desc), self.prec, ptr(self.packed), M,
+             ptr(c(xyz)), ptr(c(dirs)), ptr(c(rays_o)), ptr(c(rays_d)), ptr(c(t_vals)), spr,
+             ptr(rgb), ptr(sigma), ptr(stash), stream(self.device))
+
+    def __call__(self, inputs):
+        """model((xyz, dirs)) -> (rgb (M,3), sigma (M,1))   (model.py:94-96)."""
+        xyz, dirs = inputs
+        return MLPFunction.apply(self.params, self, xyz.contiguous(), dirs.contiguous(), None, None, None)
+
+    def forward_rays(self, rays_o: Tensor, rays_d: Tensor, t_vals: Tensor):
+        """Same as model((o + t d, d)) with the sample points generated in-kernel."""
+        return MLPFunction.apply(self.params, self, None, None, rays_o.contiguous(), rays_d.contiguous(),
+                                 t_vals.contiguous())
+
+    # -- backward ----------------------------------------------------------
+    def backward_to_params(self, M, g_rgb, g_sigma, rgb, sigma, stash) -> Tensor:
+        dz = torch.empty((M, self.dz_per_sample), dtype=torch.float32, device=self.device)
+        c = lambda t: None if t is None else t.contiguous()
+        if g_sigma is None:
+            g_sigma = torch.zeros((M, 1), dtype=torch.float32, device=self.device)
+        if self.config.l_dir and g_rgb is None:
+            g_rgb = torch.zeros((M, 3), dtype=torch.float32, device=self.device)
+        call("nerf_mlp_backward", ctypes_ref(self._desc), self.prec, ptr(self.packed), M, ptr(c(g_rgb)),
+             ptr(c(g_sigma)), ptr(c(rgb)), ptr(c(sigma)), ptr(stash), ptr(dz), stream(self.device))
+        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+        ws = int(_lib.load().nerf_mlp_wgrad_workspace_bytes(ctypes_ref(self._desc), M))
+        work = torch.empty(max(ws, 1), dtype=torch.uint8, device=self.device)
+        call("nerf_mlp_weight_grads", ctypes_ref(self._desc), self.prec, M, ptr(stash), ptr(dz), ptr(grad),
+             0, ptr(work), stream(self.device))
+        return grad
+
+
+def ctypes_ref(d):
+    import ctypes
+    return ctypes.byref(d)
+
+
+def get_coarse_or_fine_model(model_name: str, num_units: int = 256, precision: str = "fp32", seed: int = 0,
+                             device="cuda") -> NerfMLP:
+    """model.py:334-394."""
+    assert model_name in ("coarse", "fine")
+    return NerfMLP(model_name, MLPConfig(width=num_units), precision=precision, seed=seed, device=device)
+
+
+# ---------------------------------------------------------------------------
+# Optimizer: Keras Adam + ExponentialDecay (model.py:413-418)
+# ---------------------------------------------------------------------------
+def exponential_decay(step: int, initial=5e-4, decay_steps=500000, decay_rate=0.1) -> float:
+    """float32 evaluation of initial * rate ** (step / decay_steps) (non-staircase)."""
+    p = np.float32(step) / np.float32(decay_steps)
+    return float(np.float32(initial) * np.power(np.float32(decay_rate), p, dtype=np.float32))
+
+
+class AdamTF:
+    """Keras Adam (beta1 0.9, beta2 0.999, epsilon 1e-7) with TF's ApplyAdam update,
+    one fused kernel launch per parameter vector.  The per-step coefficients are
+    computed on the host in float32 like TF's _prepare_local."""
+
+    def __init__(self, models: Sequence[NerfMLP], b1=0.9, b2=0.999, eps=1e-7, schedule=exponential_decay):
+        self.models = list(models)
+        self.m = [torch.zeros_like(mm.params) for mm in self.models]
+        self.v = [torch.zeros_like(mm.params) for mm in self.models]
+        self.iterations = 0
+        self.b1, self.b2, self.eps = np.float32(b1), np.float32(b2), np.float32(eps)
+        self.schedule = schedule
+
+    def coefficients(self):
+        lr = np.float32(self.schedule(self.iterations))
+        t = np.float32(self.iterations + 1)
+        one = np.float32(1.0)
+        b1p = np.power(self.b1, t, dtype=np.float32)
+        b2p = np.power(self.b2, t, dtype=np.float32)
+        lr_t = np.float32(np.float32(lr * np.sqrt(np.float32(one - b2p), dtype=np.float32)) / np.float32(one - b1p))
+        return float(lr_t), float(np.float32(one - self.b1)), float(np.float32(one - self.b2))
+
+    def apply_gradients(self, grads: Sequence[Tensor], inv_scale: Optional[Tensor] = None,
+                        skip_flag: Optional[Tensor] = None) -> None:
+        lr_t, omb1, omb2 = self.coefficients()
+        for mm, m, v, g in zip(self.models, self.m, self.v, grads):
+            call("nerf_adam_tf", mm.n_params, ptr(mm.params), ptr(m), ptr(v), ptr(g.contiguous()),
+                 lr_t, omb1, omb2, float(self.eps), ptr(inv_scale), ptr(skip_flag), stream(mm.device))
+            mm.repack()
+        self.iterations += 1
+
+
+# ---------------------------------------------------------------------------
+# NeRF (model.py:18-237)
+# ---------------------------------------------------------------------------
+class NeRF:
+    def __init__(self, params, precision: str = "fp32", seed: int = 0, device="cuda",
+                 mlp_config: MLPConfig = MLPConfig()):
+        self.params = params
+        self.white_bg = bool(params.system.white_bg)
+        self.precision = precision
+        self.coarse_model = NerfMLP("coarse", mlp_config, precision, seed=seed, device=device)
+        self.fine_model = NerfMLP("fine", mlp_config, precision, seed=seed + 1, device=device)
+        self.optimizer = AdamTF([self.coarse_model, self.fine_model])
+
+    def forward(self, rays_o, rays_d, near, far, u_coarse=None, u_fine=None, return_indices=False):
+        """model.py:57-125 -> (post_proc_CM, post_proc_FM)."""
+        data_CM = ray_utils.create_input_batch_coarse_model(self.params, rays_o, rays_d, near, far,
+                                                            u_vals=u_coarse, materialize=False)
+        rgb_c, sig_c = self.coarse_model.forward_rays(rays_o, rays_d, data_CM["t_vals"])
+        pp_c = ray_utils.post_process_model_output(rgb_c, sig_c, data_CM["t_vals"], self.white_bg)
+        data_FM = ray_utils.create_input_batch_fine_model(self.params, rays_o, rays_d, pp_c["weights"],
+                                                          data_CM["bin_data"], data_CM["t_vals"],
+                                                          u_vals=u_fine, materialize=False,
+                                                          return_indices=return_indices)
+        rgb_f, sig_f = self.fine_model.forward_rays(rays_o, rays_d, data_FM["t_vals"])
+        pp_f = ray_utils.post_process_model_output(rgb_f, sig_f, data_FM["t_vals"], self.white_bg)
+        if return_indices:
+            pp_f["piece_idxs"] = data_FM["piece_idxs"]
+            pp_f["t_vals"] = data_FM["t_vals"]
+            pp_c["t_vals"] = data_CM["t_vals"]
+        return pp_c, pp_f
+
+    @torch.no_grad()
+    def predict(self, rays_o, rays_d, near, far, chunk: int = 32768, keep_weights: bool = False):
+        """predict_step over ray chunks (model.py:225-237, render.py:85): fine rgb/depth/acc."""
+        outs = {"pred_rgb": [], "pred_depth": [], "acc_map": []}
+        if keep_weights:
+            outs["weights"] = []
+        for i in range(0, rays_o.shape[0], chunk):
+            _, pp_f = self.forward(rays_o[i:i + chunk], rays_d[i:i + chunk], near[i:i + chunk], far[i:i + chunk])
+            for k in outs:
+                outs[k].append(pp_f[k])
+        return {k: torch.cat(v) for k, v in outs.items()}
+
+    def train_step(self, rays_o, rays_d, near, far, rgb, u_coarse=None, u_fine=None) -> Dict[str, float]:
+        """model.py:127-180: MSE(coarse) + MSE(fine), backward, Adam step."""
+        for mdl in (self.coarse_model, self.fine_model):
+            mdl.params.requires_grad_(True)
+            mdl.params.grad = None
+        pp_c, pp_f = self.forward(rays_o, rays_d, near, far, u_coarse, u_fine)
+        lc = torch.mean((pp_c["pred_rgb"] - rgb) ** 2)
+        lf = torch.mean((pp_f["pred_rgb"] - rgb) ** 2)
+        (lc + lf).backward()
+        grads = [self.coarse_model.params.grad, self.fine_model.params.grad]
+        for mdl in (self.coarse_model, self.fine_model):
+            mdl.params.requires_grad_(False)
+        self.optimizer.apply_gradients(grads)
+        return {"coarse_loss": lc.detach(), "fine_loss": lf.detach(), "pred_rgb": pp_f["pred_rgb"].detach()}
