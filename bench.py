@@ -1,0 +1,226 @@
+"""Benchmark of the NeRF ray-marching hot path on one MI355X.
+
+Workload (plain coarse+fine NeRF, 64 + 128 samples, 8x256 MLPs): one step is a
+forward render of one 800x800 view (focal 1111.1, 640,000 rays in 32,768-ray
+chunks) of the procedural Gaussian-blob scene; train throughput is measured on
+4096-ray batches drawn across 100 seed-placed views.  Inputs (rays, bounds,
+ground truth) are resident in HBM before the timed region.
+
+Prints ONE JSON line.  `value` is render rays/s of the whole job; train rays/s,
+the roofline of the dominant kernel (the fused fine-model MLP forward, timed
+with HIP events on its stream) and the CPU baseline (the oracle restatement on
+the host cores, bounded sample) ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rays/sec render & train (64+128 samples), learned vs plain at ≤0.1 dB PSNR"
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
+MLP_FLOP_PER_SAMPLE = 2 * 593408          # both dense stacks of one model, per sample
+RENDER_FLOP_PER_RAY = (64 + 192) * MLP_FLOP_PER_SAMPLE
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--train-steps", type=int, default=10)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: every GPU box here has one GPU; "
+              "the path shards by ray ranges (see DESIGN.md) but is not run on several GPUs.",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    nerf = importlib.import_module("nerf-tf2_amd")
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    sc = nerf.scene
+    scene = sc.BlobScene.from_seed(0)
+    P = nerf.Params()
+    net = nerf.NeRF(P, args.precision, seed=0, device=dev)
+
+    # ---------------- inputs resident in HBM
+    H = W = 800
+    K = sc.intrinsics(H, W, 1111.1)
+    pose = sc.random_poses(1, 1234)[0]
+    ro, rd = nerf.ray_utils.get_rays(H, W, K, pose, device=dev)
+    n_rays = ro.shape[0]
+    near = torch.full((n_rays, 1), sc.NEAR, device=dev)
+    far = torch.full((n_rays, 1), sc.FAR, device=dev)
+    chunk = 32768
+
+    # training batches: 4096 random rays across 100 seed-placed 800x800 views
+    Kt = sc.intrinsics(H, W, 1111.1)
+    poses = sc.random_poses(100, 7)
+    rng = np.random.default_rng(0)
+    batches = []
+    for b in range(4):
+        views = rng.integers(0, 100, size=4096)
+        pix = rng.integers(0, H * W, size=4096)
+        o_l, d_l = [], []
+        for v in np.unique(views):
+            o, d = nerf.ray_utils.get_rays_tf(H, W, Kt.astype(np.float32), poses[v].astype(np.float32), device=dev)
+            sel = torch.from_numpy(pix[views == v]).to(dev)
+            o_l.append(o[sel])
+            d_l.append(d[sel])
+        bo = torch.cat(o_l).contiguous()
+        bd = torch.cat(d_l).contiguous()
+        gt, _, _ = scene.render(bo.double(), bd.double(), n_samples=1024)
+        batches.append((bo, bd, torch.full((4096, 1), sc.NEAR, device=dev),
+                        torch.full((4096, 1), sc.FAR, device=dev), gt.float().contiguous()))
+    torch.cuda.synchronize()
+
+    # ---------------- render: warmup + timed steps
+    mlp_ms = []
+
+    def render(timed_kernel: bool):
+        out = []
+        for i in range(0, n_rays, chunk):
+            sl = slice(i, min(i + chunk, n_rays))
+            out.append(_render_chunk(net, nerf, ro[sl], rd[sl], near[sl], far[sl], timed_kernel, mlp_ms))
+        return out
+
+    for _ in range(args.warmup):
+        render(False)
+    torch.cuda.synchronize()
+    mlp_ms.clear()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        render(True)
+    torch.cuda.synchronize()
+    t_render = time.perf_counter() - t0
+    render_rays_s = n_rays * args.steps / t_render
+
+    # ---------------- train
+    for i in range(2):
+        net.train_step(*batches[i % 4])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.train_steps):
+        net.train_step(*batches[i % 4])
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    train_rays_s = 4096 * args.train_steps / t_train
+
+    # ---------------- roofline of the dominant kernel (fine MLP forward, full chunks)
+    full = [ms for ms, m in mlp_ms if m == chunk * 192]
+    avg_ms = float(np.mean(full)) if full else float("nan")
+    flop = chunk * 192 * MLP_FLOP_PER_SAMPLE
+    achieved = flop / (avg_ms * 1e-3) / 1e12
+    peak = FP32_PEAK_TFLOPS if args.precision == "fp32" else BF16_PEAK_TFLOPS
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": round(render_rays_s, 1), "unit": "rays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_render / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic (procedural Gaussian-blob scene, random-init weights)",
+        "config": {"workload": "plain coarse+fine NeRF render of one 800x800 view (640,000 rays, 32,768-ray "
+                               "chunks, 64+128 samples, 8x256 MLPs); train on 4096-ray batches",
+                   "rays_per_step": n_rays, "chunk": chunk, "train_batch": 4096},
+        "train_rays_per_s": round(train_rays_s, 1),
+        "train_ms_per_step": round(1e3 * t_train / args.train_steps, 3),
+        "render_tflops": round(render_rays_s * RENDER_FLOP_PER_RAY / 1e12, 2),
+        "roofline": {"bound": "mfma", "kernel": "mlp_fwd (fine model, 32,768 rays x 192 samples per launch)",
+                     "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 3), "flop_per_launch": flop},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def _render_chunk(net, nerf, ro, rd, near, far, timed_kernel, mlp_ms):
+    """NeRF.forward for one chunk with the fine MLP launch bracketed by events."""
+    import torch
+    ru = nerf.ray_utils
+    with torch.no_grad():
+        dc = ru.create_input_batch_coarse_model(net.params, ro, rd, near, far, materialize=False)
+        rgb_c, sig_c = net.coarse_model.forward_rays(ro, rd, dc["t_vals"])
+        pc = ru.post_process_model_output(rgb_c, sig_c, dc["t_vals"], net.white_bg)
+        df = ru.create_input_batch_fine_model(net.params, ro, rd, pc["weights"], dc["bin_data"], dc["t_vals"],
+                                              materialize=False)
+        if timed_kernel:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+        rgb_f, sig_f = net.fine_model.forward_rays(ro, rd, df["t_vals"])
+        if timed_kernel:
+            e1.record()
+        pf = ru.post_process_model_output(rgb_f, sig_f, df["t_vals"], net.white_bg)
+        if timed_kernel:
+            e1.synchronize()
+            mlp_ms.append((e0.elapsed_time(e1), int(df["t_vals"].numel())))
+    return pf
+
+
+def cpu_baseline(seconds: float):
+    """The oracle (PyTorch-CPU float32 restatement of the reference path) on the
+    host cores: forward of a bounded ray sample of the same render workload."""
+    import numpy as np
+    import torch
+    oracle = importlib.import_module("oracle.restatement")
+    sc = importlib.import_module("nerf-tf2_amd").scene
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    spec = oracle.MLPSpec()
+    wc = [torch.from_numpy(w) for w in oracle.glorot_weights(spec, 0)]
+    wf = [torch.from_numpy(w) for w in oracle.glorot_weights(spec, 1)]
+    K = sc.intrinsics(800, 800, 1111.1)
+    ro, rd = oracle.get_rays(800, 800, K, sc.random_poses(1, 1234)[0])
+    P = oracle.make_params()
+    n = 512
+    done = 0
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        while True:
+            sl = slice(done % 600000, done % 600000 + n)
+            o = torch.from_numpy(np.ascontiguousarray(ro[sl], dtype=np.float32))
+            d = torch.from_numpy(np.ascontiguousarray(rd[sl], dtype=np.float32))
+            oracle.forward(P, spec, wc, wf, o, d, torch.full((n, 1), 2.0), torch.full((n, 1), 6.0),
+                           torch.rand(n, 64), torch.rand(n, 128))
+            done += n
+            if time.perf_counter() - t0 > seconds:
+                break
+    dt = time.perf_counter() - t0
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(done / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{done} rays of the 800x800 render workload (coarse+fine forward, {n}-ray chunks, "
+                      f"{dt:.1f} s, torch-CPU float32 restatement, {cpu_model})"}
+
+
+if __name__ == "__main__":
+    main()

@@ -93,7 +93,8 @@ int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse, int32_t n
 /* ------------------------------------------------------------------------
  * post_process_model_output (nerf/utils/ray_utils.py:484-551) with
  * compute_weights (:426-482) and sigma_to_alpha (:408-424).
- *   sigma (N*S,1), sample_rgb (N*S,3), t_vals (N,S) -> weights (N,S),
+ *   sigma (N*S,1), sample_rgb (N*S,3) (NULL: weights only, pred_rgb = white
+ *   background or 0), t_vals (N,S) -> weights (N,S),
  *   pred_rgb (N,3), pred_depth (N,), acc_map (N,).  Transmittance is an
  *   exclusive, compensated log-space scan over one wave per ray.
  *   The _f64 variant takes and returns float64 (gradient checks).
@@ -200,6 +201,43 @@ int32_t nerf_adam_tf(int64_t n, float* var, float* m, float* v, const float* gra
 /* Sets *flag (device int32) to 1 if any of the n values is inf or NaN
  * (dynamic loss scaling), else leaves it unchanged. */
 int32_t nerf_check_finite(int64_t n, const float* x, int32_t* flag, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Float64 layer kernels for the finite-difference check of the hand-written
+ * backward (the FP64 gradient check; no counterpart in the reference, whose
+ * gradients come from TF's GradientTape over model.py:355-392).  Row-major
+ * float64; kernels are Keras (fan_in, fan_out).
+ *   encode:      out (M, 3+6L) = PositionalEncoder (model.py:319-330) in double
+ *   dense fwd:   Y (M,N) = act([X1 (M,K1), X2 (M,K2)] @ kernel + bias),
+ *                activation 0 identity, 1 ReLU, 2 sigmoid; X2 may be NULL (K2 = 0)
+ *   dense dX:    dX (M,K) = dZ (M,N) @ kernel[:K]^T, times [X_hidden > 0] if relu_mask
+ *   dense dW:    d_kernel[row0:row0+K] = X^T dZ; d_bias = sum_m dZ (NULL: skipped)
+ * ---------------------------------------------------------------------- */
+int32_t nerf_f64_encode(int64_t M, int32_t L, const double* x, double* out, void* stream);
+int32_t nerf_f64_dense_forward(int64_t M, int32_t K1, const double* X1, int32_t K2,
+                               const double* X2, int32_t N, const double* kernel,
+                               const double* bias, int32_t activation, double* Y, void* stream);
+int32_t nerf_f64_dense_backward_dx(int64_t M, int32_t N, const double* dZ, int32_t K,
+                                   const double* kernel, const double* X_hidden, int32_t relu_mask,
+                                   double* dX, void* stream);
+int32_t nerf_f64_dense_wgrad(int64_t M, int32_t K, const double* X, int32_t N, const double* dZ,
+                             int32_t row0, double* d_kernel, double* d_bias, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned proposal (an addition with no counterpart in the reference: it
+ * replaces the coarse pass of NeRF.forward, model.py:88-103, for placing the
+ * fine samples).  Histogram upper-bound loss of mip-NeRF 360 between the
+ * proposal weights on its bins (prop_edges (N, n_prop+1), prop_weights
+ * (N, n_prop)) and the fine weights fine_weights (N, n_fine) at the sorted
+ * sample positions t_fine (N, n_fine); the last fine interval ends at far (N,1).
+ * Outputs (each may be NULL): missed (N,) missed weight mass per ray, loss (N,)
+ * per-ray loss, g_prop_weights (N, n_prop) = grad_scale * d loss / d prop_weights.
+ * ---------------------------------------------------------------------- */
+int32_t nerf_proposal_interlevel(int64_t n_rays, int32_t n_prop, int32_t n_fine,
+                                 const float* prop_edges, const float* prop_weights,
+                                 const float* t_fine, const float* fine_weights, const float* far,
+                                 float* missed, float* loss, float* g_prop_weights,
+                                 float grad_scale, void* stream);
 
 #ifdef __cplusplus
 }

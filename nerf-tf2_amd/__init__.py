@@ -8,7 +8,8 @@ a hyphen).  Modules:
   _lib       ctypes binding of libnerf_hip.so (include/nerf_hip.h)
 """
 from . import _lib, ray_utils, model, scene  # noqa: F401
-from .model import MLPConfig, NeRF, NerfMLP, AdamTF, get_coarse_or_fine_model  # noqa: F401
+from .model import MLPConfig, NeRF, NerfMLP, AdamTF, DynamicLossScaler, get_coarse_or_fine_model  # noqa: F401
+from .proposal import ProposalNeRF  # noqa: F401
 
 __all__ = ["ray_utils", "model", "scene", "MLPConfig", "NeRF", "NerfMLP", "AdamTF",
            "get_coarse_or_fine_model", "Params", "make_params"]

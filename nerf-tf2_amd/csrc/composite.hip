@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void composite_fwd_kernel(
   if (ray >= n_rays) return;  // wave-uniform
   const T* sig_r = sigma + ray * S;
   const T* t_r = t_vals + ray * S;
-  const T* rgb_r = rgb + ray * S * 3;
+  const T* rgb_r = rgb ? rgb + ray * S * 3 : nullptr;
   RayState<T> st;
   ray_forward<T, P>(sig_r, t_r, S, lane, st);
   T cr = T(0), cg = T(0), cb = T(0), dep = T(0), acc = T(0);
@@ -119,9 +119,11 @@ __global__ __launch_bounds__(256) void composite_fwd_kernel(
     if (i < S) {
       const T w = fmul(st.alpha[p], st.Tr[p]);  // ray_utils.py:480
       weights[ray * S + i] = w;
-      cr = fadd(cr, fmul(w, rgb_r[3 * i + 0]));  // ray_utils.py:534
-      cg = fadd(cg, fmul(w, rgb_r[3 * i + 1]));
-      cb = fadd(cb, fmul(w, rgb_r[3 * i + 2]));
+      if (rgb) {
+        cr = fadd(cr, fmul(w, rgb_r[3 * i + 0]));  // ray_utils.py:534
+        cg = fadd(cg, fmul(w, rgb_r[3 * i + 1]));
+        cb = fadd(cb, fmul(w, rgb_r[3 * i + 2]));
+      }
       dep = fadd(dep, fmul(st.t[p], w));  // ray_utils.py:537
       acc = fadd(acc, w);                 // ray_utils.py:540
     }
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
   if (ray >= n_rays) return;
   const T* sig_r = sigma + ray * S;
   const T* t_r = t_vals + ray * S;
-  const T* rgb_r = rgb + ray * S * 3;
+  const T* rgb_r = rgb ? rgb + ray * S * 3 : nullptr;
   RayState<T> st;
   ray_forward<T, P>(sig_r, t_r, S, lane, st);
   const T gr = g_rgb[3 * ray + 0], gg = g_rgb[3 * ray + 1], gb = g_rgb[3 * ray + 2];
@@ -178,7 +180,8 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
     w[p] = fmul(st.alpha[p], st.Tr[p]);
     G[p] = T(0);
     if (i < S) {
-      T g = fadd(fadd(fmul(gr, rgb_r[3 * i]), fmul(gg, rgb_r[3 * i + 1])), fmul(gb, rgb_r[3 * i + 2]));
+      T g = rgb_r ? fadd(fadd(fmul(gr, rgb_r[3 * i]), fmul(gg, rgb_r[3 * i + 1])), fmul(gb, rgb_r[3 * i + 2]))
+                  : T(0);
       g = fsub(g, gbg);
       g = fadd(g, fmul(gd, st.t[p]));
       g = fadd(g, ga);
@@ -202,9 +205,11 @@ __global__ __launch_bounds__(256) void composite_bwd_kernel(
     if (i < S) {
       const T da = fsub(fmul(G[p], st.Tr[p]), fdiv(R, st.f[p]));
       d_sigma[ray * S + i] = fmul(fmul(da, st.dlt[p]), st.e[p]);
-      d_rgb[(ray * S + i) * 3 + 0] = fmul(gr, w[p]);
-      d_rgb[(ray * S + i) * 3 + 1] = fmul(gg, w[p]);
-      d_rgb[(ray * S + i) * 3 + 2] = fmul(gb, w[p]);
+      if (d_rgb) {
+        d_rgb[(ray * S + i) * 3 + 0] = fmul(gr, w[p]);
+        d_rgb[(ray * S + i) * 3 + 1] = fmul(gg, w[p]);
+        d_rgb[(ray * S + i) * 3 + 2] = fmul(gb, w[p]);
+      }
     }
     R = fadd(R, fmul(G[p], w[p]));
   }
@@ -216,7 +221,7 @@ int32_t launch_fwd(int64_t n_rays, int S, int white_bg, const T* sigma, const T*
                    void* stream, const char* name) {
   NERF_CHECK_ARG(n_rays >= 0 && S >= 1 && S <= 64 * kMaxP, "%s: S=%d must be in [1, %d]", name,
                  S, 64 * kMaxP);
-  NERF_CHECK_ARG(sigma && rgb && t_vals && weights && pred_rgb && pred_depth && acc_map,
+  NERF_CHECK_ARG(sigma && t_vals && weights && pred_rgb && pred_depth && acc_map,
                  "%s: null pointer", name);
   if (n_rays == 0) return NERF_OK;
   const int P = (S + 63) / 64;
@@ -248,7 +253,7 @@ int32_t launch_bwd(int64_t n_rays, int S, int white_bg, const T* sigma, const T*
                    const T* g_w, T* d_sigma, T* d_rgb, void* stream, const char* name) {
   NERF_CHECK_ARG(n_rays >= 0 && S >= 1 && S <= 64 * kMaxP, "%s: S=%d must be in [1, %d]", name,
                  S, 64 * kMaxP);
-  NERF_CHECK_ARG(sigma && rgb && t_vals && g_rgb && d_sigma && d_rgb, "%s: null pointer", name);
+  NERF_CHECK_ARG(sigma && t_vals && g_rgb && d_sigma && (d_rgb || !rgb), "%s: null pointer", name);
   if (n_rays == 0) return NERF_OK;
   const int P = (S + 63) / 64;
   dim3 grid((unsigned)((n_rays + 3) / 4)), block(256);

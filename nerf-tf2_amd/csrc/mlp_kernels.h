@@ -275,3 +275,129 @@ __device__ __forceinline__ void rgb_head_f32(const floatx16 (&accr)[NBR], const 
   rgb[1] = sigmoidf(p1 + bias[1]);
   rgb[2] = sigmoidf(p2 + bias[2]);
 }
+
+// ---------------------------------------------------------------------------
+// bf16 building blocks (v_mfma_f32_32x32x16_bf16: 8 bf16 of A and B per lane)
+// ---------------------------------------------------------------------------
+typedef float floatx8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ floatx16 mfma_bf16(bf16x8 a, bf16x8 b, floatx16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 to_bf16x8(float a0, float a1, float a2, float a3, float a4,
+                                            float a5, float a6, float a7) {
+  const floatx8 v = {a0, a1, a2, a3, a4, a5, a6, a7};
+  return __builtin_convertvector(v, bf16x8);
+}
+
+// k-step t of a 32-feature block: accumulators 8h .. 8h+7 (h = t % 2).
+__device__ __forceinline__ bf16x8 acc_to_bf16(const floatx16& a, int h) {
+  return to_bf16x8(a[8 * h], a[8 * h + 1], a[8 * h + 2], a[8 * h + 3], a[8 * h + 4], a[8 * h + 5],
+                   a[8 * h + 6], a[8 * h + 7]);
+}
+
+template <int K>
+__device__ __forceinline__ void slots_to_bf16(const float (&v)[K], bf16x8 (&out)[K / 8]) {
+#pragma unroll
+  for (int t = 0; t < K / 8; ++t)
+    out[t] = to_bf16x8(v[8 * t], v[8 * t + 1], v[8 * t + 2], v[8 * t + 3], v[8 * t + 4], v[8 * t + 5],
+                       v[8 * t + 6], v[8 * t + 7]);
+}
+
+// acc[nb] += A * B over KT register k-steps (two weight k-steps in flight).
+template <int NBo, int KT>
+__device__ __forceinline__ void mfma_regs_bf16(floatx16 (&acc)[NBo], const bf16x8* __restrict__ wq,
+                                               const bf16x8 (&b)[KT], int lane) {
+  bf16x8 wa[NBo], wb[NBo];
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[nb * 64 + lane];
+#pragma unroll
+  for (int t = 0; t < KT; t += 2) {
+    if (t + 1 < KT) {
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) wb[nb] = wq[((t + 1) * NBo + nb) * 64 + lane];
+    }
+#pragma unroll
+    for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_bf16(wa[nb], b[t], acc[nb]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (t + 2 < KT) {
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[((t + 2) * NBo + nb) * 64 + lane];
+    }
+    if (t + 1 < KT) {
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_bf16(wb[nb], b[t + 1], acc[nb]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// acc[nb] += A * B over T k-steps whose B operand sits in the wave's LDS slab
+// act[t][lane] (bf16x8); weights prefetched two k-steps ahead.
+template <int NBo, int T>
+__device__ __forceinline__ void mfma_lds_bf16(floatx16 (&acc)[NBo], const bf16x8* __restrict__ wq,
+                                              const bf16x8* act, int lane) {
+  static_assert(T % 2 == 0, "T must be even");
+  bf16x8 wa[NBo], wb[NBo];
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[nb * 64 + lane];
+  for (int t = 0; t < T; t += 2) {
+#pragma unroll
+    for (int nb = 0; nb < NBo; ++nb) wb[nb] = wq[((t + 1) * NBo + nb) * 64 + lane];
+    bf16x8 b = act[t * 64 + lane];
+#pragma unroll
+    for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_bf16(wa[nb], b, acc[nb]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (t + 2 < T) {
+#pragma unroll
+      for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[((t + 2) * NBo + nb) * 64 + lane];
+    }
+    b = act[(t + 1) * 64 + lane];
+#pragma unroll
+    for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_bf16(wb[nb], b, acc[nb]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Accumulators -> bf16 B-operand slab: k-step 2 nb + h at act[(2nb+h)][lane].
+template <int NBo>
+__device__ __forceinline__ void store_act_bf16(const floatx16 (&acc)[NBo], bf16x8* act, int lane) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb) {
+    act[(2 * nb) * 64 + lane] = acc_to_bf16(acc[nb], 0);
+    act[(2 * nb + 1) * 64 + lane] = acc_to_bf16(acc[nb], 1);
+  }
+}
+
+// Accumulators -> a row-major bf16 feature row (natural order), 4 features per store.
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4_ __attribute__((ext_vector_type(4)));
+template <int NBo>
+__device__ __forceinline__ void store_rows_bf16(const floatx16 (&acc)[NBo], __bf16* row, int g) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const floatx4_ v = {acc[nb][4 * a], acc[nb][4 * a + 1], acc[nb][4 * a + 2], acc[nb][4 * a + 3]};
+      *reinterpret_cast<bf16x4*>(row + 32 * nb + 8 * a + 4 * g) = __builtin_convertvector(v, bf16x4);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void store_slots_bf16(const float (&v)[K], __bf16* dst) {
+#pragma unroll
+  for (int e = 0; e < K; e += 4) {
+    const floatx4_ x = {v[e], v[e + 1], v[e + 2], v[e + 3]};
+    *reinterpret_cast<bf16x4*>(dst + e) = __builtin_convertvector(x, bf16x4);
+  }
+}
+
+// Round the accumulators through bf16 (the values the next layer actually sees).
+template <int NBo>
+__device__ __forceinline__ void round_acc_bf16(floatx16 (&acc)[NBo]) {
+#pragma unroll
+  for (int nb = 0; nb < NBo; ++nb)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[nb][i] = (float)(__bf16)acc[nb][i];
+}

@@ -219,10 +219,15 @@ def get_coarse_or_fine_model(model_name: str, num_units: int = 256, precision: s
 # ---------------------------------------------------------------------------
 # Optimizer: Keras Adam + ExponentialDecay (model.py:413-418)
 # ---------------------------------------------------------------------------
+def _pow_f32(base, e) -> np.float32:
+    """float32 base ** e, evaluated in float64 and rounded once."""
+    return np.float32(np.float64(np.float32(base)) ** np.float64(e))
+
+
 def exponential_decay(step: int, initial=5e-4, decay_steps=500000, decay_rate=0.1) -> float:
-    """float32 evaluation of initial * rate ** (step / decay_steps) (non-staircase)."""
+    """initial * rate ** (step / decay_steps) with float32 operands (non-staircase)."""
     p = np.float32(step) / np.float32(decay_steps)
-    return float(np.float32(initial) * np.power(np.float32(decay_rate), p, dtype=np.float32))
+    return float(np.float32(np.float32(initial) * _pow_f32(decay_rate, p)))
 
 
 class AdamTF:
@@ -242,8 +247,8 @@ class AdamTF:
         lr = np.float32(self.schedule(self.iterations))
         t = np.float32(self.iterations + 1)
         one = np.float32(1.0)
-        b1p = np.power(self.b1, t, dtype=np.float32)
-        b2p = np.power(self.b2, t, dtype=np.float32)
+        b1p = _pow_f32(self.b1, t)
+        b2p = _pow_f32(self.b2, t)
         lr_t = np.float32(np.float32(lr * np.sqrt(np.float32(one - b2p), dtype=np.float32)) / np.float32(one - b1p))
         return float(lr_t), float(np.float32(one - self.b1)), float(np.float32(one - self.b2))
 
@@ -258,35 +263,91 @@ class AdamTF:
 
 
 # ---------------------------------------------------------------------------
+# Dynamic loss scaling (bf16 training)
+# ---------------------------------------------------------------------------
+class DynamicLossScaler:
+    """Scales the loss by S before the backward, checks every gradient for
+    inf/NaN on the device, lets Adam unscale by 1/S (device scalar) and skip the
+    step on overflow; S halves on overflow and doubles after `interval` clean
+    steps."""
+
+    def __init__(self, device, init_scale=2.0 ** 15, interval=2000, factor=2.0, min_scale=1.0):
+        self.scale = float(init_scale)
+        self.interval, self.factor, self.min_scale = interval, factor, min_scale
+        self.good = 0
+        self.skipped = 0
+        self.flag = torch.zeros(1, dtype=torch.int32, device=device)
+        self.inv = torch.ones(1, dtype=torch.float32, device=device)
+
+    def prepare(self, grads: Sequence[Tensor]):
+        self.flag.zero_()
+        self.inv.fill_(1.0 / self.scale)
+        for g in grads:
+            call("nerf_check_finite", g.numel(), ptr(g), ptr(self.flag), stream(g.device))
+        return self.inv, self.flag
+
+    def update(self) -> bool:
+        """Host-side bookkeeping after the step (reads the overflow flag)."""
+        overflow = bool(int(self.flag.item()))
+        if overflow:
+            self.scale = max(self.scale / self.factor, self.min_scale)
+            self.good = 0
+            self.skipped += 1
+        else:
+            self.good += 1
+            if self.good >= self.interval:
+                self.scale *= self.factor
+                self.good = 0
+        return overflow
+
+
+# ---------------------------------------------------------------------------
 # NeRF (model.py:18-237)
 # ---------------------------------------------------------------------------
+def coarse_fine_forward(params, coarse: NerfMLP, fine: NerfMLP, rays_o, rays_d, near, far, white_bg: bool,
+                        u_coarse=None, u_fine=None, return_indices=False):
+    """NeRF.forward (model.py:57-125) on the fused kernels."""
+    data_CM = ray_utils.create_input_batch_coarse_model(params, rays_o, rays_d, near, far,
+                                                        u_vals=u_coarse, materialize=False)
+    rgb_c, sig_c = coarse.forward_rays(rays_o, rays_d, data_CM["t_vals"])
+    pp_c = ray_utils.post_process_model_output(rgb_c, sig_c, data_CM["t_vals"], white_bg)
+    data_FM = ray_utils.create_input_batch_fine_model(params, rays_o, rays_d, pp_c["weights"],
+                                                      data_CM["bin_data"], data_CM["t_vals"],
+                                                      u_vals=u_fine, materialize=False,
+                                                      return_indices=return_indices)
+    rgb_f, sig_f = fine.forward_rays(rays_o, rays_d, data_FM["t_vals"])
+    pp_f = ray_utils.post_process_model_output(rgb_f, sig_f, data_FM["t_vals"], white_bg)
+    if return_indices:
+        pp_f["piece_idxs"] = data_FM["piece_idxs"]
+        pp_f["t_vals"] = data_FM["t_vals"]
+        pp_c["t_vals"] = data_CM["t_vals"]
+    return pp_c, pp_f
+
+
 class NeRF:
+    """Plain coarse + fine NeRF (model.py:18-237).  precision "fp32" is the
+    reference form; "bf16" runs the MLP matmuls in bf16 with fp32 master
+    weights and dynamic loss scaling."""
+
     def __init__(self, params, precision: str = "fp32", seed: int = 0, device="cuda",
                  mlp_config: MLPConfig = MLPConfig()):
         self.params = params
         self.white_bg = bool(params.system.white_bg)
         self.precision = precision
+        self.device = torch.device(device)
         self.coarse_model = NerfMLP("coarse", mlp_config, precision, seed=seed, device=device)
         self.fine_model = NerfMLP("fine", mlp_config, precision, seed=seed + 1, device=device)
         self.optimizer = AdamTF([self.coarse_model, self.fine_model])
+        self.scaler = DynamicLossScaler(self.device) if precision == "bf16" else None
+
+    @property
+    def models(self):
+        return [self.coarse_model, self.fine_model]
 
     def forward(self, rays_o, rays_d, near, far, u_coarse=None, u_fine=None, return_indices=False):
         """model.py:57-125 -> (post_proc_CM, post_proc_FM)."""
-        data_CM = ray_utils.create_input_batch_coarse_model(self.params, rays_o, rays_d, near, far,
-                                                            u_vals=u_coarse, materialize=False)
-        rgb_c, sig_c = self.coarse_model.forward_rays(rays_o, rays_d, data_CM["t_vals"])
-        pp_c = ray_utils.post_process_model_output(rgb_c, sig_c, data_CM["t_vals"], self.white_bg)
-        data_FM = ray_utils.create_input_batch_fine_model(self.params, rays_o, rays_d, pp_c["weights"],
-                                                          data_CM["bin_data"], data_CM["t_vals"],
-                                                          u_vals=u_fine, materialize=False,
-                                                          return_indices=return_indices)
-        rgb_f, sig_f = self.fine_model.forward_rays(rays_o, rays_d, data_FM["t_vals"])
-        pp_f = ray_utils.post_process_model_output(rgb_f, sig_f, data_FM["t_vals"], self.white_bg)
-        if return_indices:
-            pp_f["piece_idxs"] = data_FM["piece_idxs"]
-            pp_f["t_vals"] = data_FM["t_vals"]
-            pp_c["t_vals"] = data_CM["t_vals"]
-        return pp_c, pp_f
+        return coarse_fine_forward(self.params, self.coarse_model, self.fine_model, rays_o, rays_d, near, far,
+                                   self.white_bg, u_coarse, u_fine, return_indices)
 
     @torch.no_grad()
     def predict(self, rays_o, rays_d, near, far, chunk: int = 32768, keep_weights: bool = False):
@@ -300,17 +361,33 @@ class NeRF:
                 outs[k].append(pp_f[k])
         return {k: torch.cat(v) for k, v in outs.items()}
 
-    def train_step(self, rays_o, rays_d, near, far, rgb, u_coarse=None, u_fine=None) -> Dict[str, float]:
-        """model.py:127-180: MSE(coarse) + MSE(fine), backward, Adam step."""
-        for mdl in (self.coarse_model, self.fine_model):
-            mdl.params.requires_grad_(True)
+    def _step(self, total: Tensor) -> None:
+        """Backward of `total`, then Adam (with loss scaling in bf16)."""
+        models = self.models
+        for mdl in models:
             mdl.params.grad = None
+        if self.scaler is None:
+            total.backward()
+            grads = [mdl.params.grad for mdl in models]
+            for mdl in models:
+                mdl.params.requires_grad_(False)
+            self.optimizer.apply_gradients(grads)
+        else:
+            (total * self.scaler.scale).backward()
+            grads = [mdl.params.grad for mdl in models]
+            for mdl in models:
+                mdl.params.requires_grad_(False)
+            inv, flag = self.scaler.prepare(grads)
+            self.optimizer.apply_gradients(grads, inv_scale=inv, skip_flag=flag)
+            if self.scaler.update():
+                self.optimizer.iterations -= 1  # a skipped step does not advance Adam
+
+    def train_step(self, rays_o, rays_d, near, far, rgb, u_coarse=None, u_fine=None) -> Dict[str, Tensor]:
+        """model.py:127-180: MSE(coarse) + MSE(fine), backward, Adam step."""
+        for mdl in self.models:
+            mdl.params.requires_grad_(True)
         pp_c, pp_f = self.forward(rays_o, rays_d, near, far, u_coarse, u_fine)
-        lc = torch.mean((pp_c["pred_rgb"] - rgb) ** 2)
-        lf = torch.mean((pp_f["pred_rgb"] - rgb) ** 2)
-        (lc + lf).backward()
-        grads = [self.coarse_model.params.grad, self.fine_model.params.grad]
-        for mdl in (self.coarse_model, self.fine_model):
-            mdl.params.requires_grad_(False)
-        self.optimizer.apply_gradients(grads)
+        lc = ray_utils.mse_loss(rgb, pp_c["pred_rgb"])
+        lf = ray_utils.mse_loss(rgb, pp_f["pred_rgb"])
+        self._step(lc + lf)
         return {"coarse_loss": lc.detach(), "fine_loss": lf.detach(), "pred_rgb": pp_f["pred_rgb"].detach()}

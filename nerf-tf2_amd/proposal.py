@@ -1,0 +1,155 @@
+"""Learned-proposal form of the NeRF hot path.
+
+An addition with no counterpart in the reference: a small density MLP (4 ReLU
+layers x 64 on an L = 8 xyz encoding, softplus density) replaces the coarse
+8x256 pass of NeRF.forward (model.py:88-103) for placing the fine samples.
+
+Per ray:
+  1. 48 stratified proposal samples (the coarse sampler with 48 bins);
+  2. proposal densities -> weights (the compositing kernel without colours);
+  3. 128 fine samples from the proposal weights (the same CDF resampler);
+  4. the fine 8x256 model on the 48 + 128 sorted samples -> rgb / depth / acc;
+  5. the proposal's missed weight mass against the fine weights (interlevel
+     kernel).  Rays missing more than `fallback_threshold` are re-rendered by
+     the plain coarse + fine pass (64 + 128 samples) on a compacted subset.
+Training: the fine model on its photometric loss; the proposal online on the
+histogram upper-bound loss against the detached fine weights; the coarse
+model (the fallback) on its photometric loss over a strided subset of the
+batch (`coarse_stride`).
+"""
+from __future__ import annotations
+
+from typing import Dict, Optional
+
+import torch
+
+from . import ray_utils
+from .model import AdamTF, DynamicLossScaler, MLPConfig, NerfMLP, coarse_fine_forward
+
+Tensor = torch.Tensor
+
+PROPOSAL_CONFIG = MLPConfig(depth=4, width=64, skip=-1, l_xyz=8, l_dir=0)
+
+
+class _P:
+    """params-like object for the samplers."""
+
+    def __init__(self, n_coarse, n_fine, lin_inv_depth=True, white_bg=True):
+        class S:  # noqa: N801
+            pass
+        self.sampling = S()
+        self.sampling.N_coarse, self.sampling.N_fine = n_coarse, n_fine
+        self.sampling.perturb, self.sampling.lin_inv_depth = True, lin_inv_depth
+        self.system = S()
+        self.system.white_bg = white_bg
+
+
+class ProposalNeRF:
+    def __init__(self, params, precision: str = "bf16", seed: int = 0, device="cuda", n_prop: int = 48,
+                 fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 1.0):
+        self.params = params
+        self.white_bg = bool(params.system.white_bg)
+        self.precision = precision
+        self.device = torch.device(device)
+        self.n_prop = n_prop
+        self.n_fine = int(params.sampling.N_fine)
+        self.fallback_threshold = fallback_threshold
+        self.coarse_stride = coarse_stride
+        self.interlevel_weight = interlevel_weight
+        self.p_prop = _P(n_prop, self.n_fine, params.sampling.lin_inv_depth, self.white_bg)
+        self.proposal = NerfMLP("proposal", PROPOSAL_CONFIG, precision, seed=seed + 2, device=device)
+        self.coarse_model = NerfMLP("coarse", MLPConfig(), precision, seed=seed, device=device)
+        self.fine_model = NerfMLP("fine", MLPConfig(), precision, seed=seed + 1, device=device)
+        self.optimizer = AdamTF(self.models)
+        self.scaler = DynamicLossScaler(self.device) if precision == "bf16" else None
+        self.last_fallback_fraction = 0.0
+
+    @property
+    def models(self):
+        return [self.proposal, self.coarse_model, self.fine_model]
+
+    # ------------------------------------------------------------------
+    def _proposal_fine(self, rays_o, rays_d, near, far, u_prop=None, u_fine=None):
+        dp = ray_utils.create_input_batch_coarse_model(self.p_prop, rays_o, rays_d, near, far, u_vals=u_prop,
+                                                       materialize=False)
+        sig_p = self.proposal.forward_rays(rays_o, rays_d, dp["t_vals"])
+        pp_p = ray_utils.post_process_model_output(None, sig_p, dp["t_vals"], False)
+        df = ray_utils.create_input_batch_fine_model(self.p_prop, rays_o, rays_d, pp_p["weights"], dp["bin_data"],
+                                                     dp["t_vals"], u_vals=u_fine, materialize=False)
+        rgb_f, sig_f = self.fine_model.forward_rays(rays_o, rays_d, df["t_vals"])
+        pp_f = ray_utils.post_process_model_output(rgb_f, sig_f, df["t_vals"], self.white_bg)
+        return dp, pp_p, df, pp_f
+
+    def forward(self, rays_o, rays_d, near, far, fallback: bool = True, u_prop=None, u_fine=None):
+        """Returns (post_proc_proposal, post_proc_FM, missed).  With fallback, the
+        rgb / depth / acc of rays whose missed mass exceeds the threshold come from
+        the plain coarse + fine pass (their weights stay those of the proposal pass)."""
+        dp, pp_p, df, pp_f = self._proposal_fine(rays_o, rays_d, near, far, u_prop, u_fine)
+        missed, _, _ = ray_utils.interlevel(dp["bin_data"]["bin_edges"], pp_p["weights"], df["t_vals"],
+                                            pp_f["weights"], far)
+        pp_f["t_vals"] = df["t_vals"]
+        if fallback:
+            idx = torch.nonzero(missed > self.fallback_threshold).flatten()
+            self.last_fallback_fraction = idx.numel() / max(1, rays_o.shape[0])
+            if idx.numel():
+                o, d = rays_o.index_select(0, idx), rays_d.index_select(0, idx)
+                nr, fr = near.index_select(0, idx), far.index_select(0, idx)
+                _, pf2 = coarse_fine_forward(self.params, self.coarse_model, self.fine_model, o, d, nr, fr,
+                                             self.white_bg)
+                for k in ("pred_rgb", "pred_depth", "acc_map"):
+                    pp_f[k] = pp_f[k].index_copy(0, idx, pf2[k])
+        return pp_p, pp_f, missed
+
+    @torch.no_grad()
+    def predict(self, rays_o, rays_d, near, far, chunk: int = 32768, fallback: bool = True):
+        outs = {"pred_rgb": [], "pred_depth": [], "acc_map": []}
+        n_fb = 0
+        for i in range(0, rays_o.shape[0], chunk):
+            _, pp_f, _ = self.forward(rays_o[i:i + chunk], rays_d[i:i + chunk], near[i:i + chunk],
+                                      far[i:i + chunk], fallback=fallback)
+            n_fb += int(round(self.last_fallback_fraction * min(chunk, rays_o.shape[0] - i)))
+            for k in outs:
+                outs[k].append(pp_f[k])
+        self.last_fallback_fraction = n_fb / max(1, rays_o.shape[0])
+        return {k: torch.cat(v) for k, v in outs.items()}
+
+    # ------------------------------------------------------------------
+    def train_step(self, rays_o, rays_d, near, far, rgb) -> Dict[str, Tensor]:
+        for mdl in self.models:
+            mdl.params.requires_grad_(True)
+            mdl.params.grad = None
+        n = rays_o.shape[0]
+        dp, pp_p, df, pp_f = self._proposal_fine(rays_o, rays_d, near, far)
+        lf = ray_utils.mse_loss(rgb, pp_f["pred_rgb"])
+        # proposal: d(mean interlevel loss)/d w_p from the kernel, applied through a surrogate
+        missed, il, g_wp = ray_utils.interlevel(dp["bin_data"]["bin_edges"], pp_p["weights"], df["t_vals"],
+                                                pp_f["weights"], far, want_grad=True,
+                                                grad_scale=self.interlevel_weight / n)
+        surrogate = (pp_p["weights"] * g_wp).sum()
+        # coarse (fallback) model on a strided subset of the batch
+        sub = slice(0, n, self.coarse_stride)
+        o, d, nr, fr = rays_o[sub].contiguous(), rays_d[sub].contiguous(), near[sub].contiguous(), far[sub].contiguous()
+        dc = ray_utils.create_input_batch_coarse_model(self.params, o, d, nr, fr, materialize=False)
+        rgb_c, sig_c = self.coarse_model.forward_rays(o, d, dc["t_vals"])
+        pp_c = ray_utils.post_process_model_output(rgb_c, sig_c, dc["t_vals"], self.white_bg)
+        lc = ray_utils.mse_loss(rgb[sub].contiguous(), pp_c["pred_rgb"])
+        total = lf + lc + surrogate
+        models = self.models
+        if self.scaler is None:
+            total.backward()
+            grads = [m.params.grad if m.params.grad is not None else torch.zeros_like(m.params) for m in models]
+            for m in models:
+                m.params.requires_grad_(False)
+            self.optimizer.apply_gradients(grads)
+        else:
+            (total * self.scaler.scale).backward()
+            grads = [m.params.grad if m.params.grad is not None else torch.zeros_like(m.params) for m in models]
+            for m in models:
+                m.params.requires_grad_(False)
+            inv, flag = self.scaler.prepare(grads)
+            self.optimizer.apply_gradients(grads, inv_scale=inv, skip_flag=flag)
+            if self.scaler.update():
+                self.optimizer.iterations -= 1
+        return {"fine_loss": lf.detach(), "coarse_loss": lc.detach(), "interlevel": il.mean().detach(),
+                "missed_mean": missed.mean().detach(),
+                "fallback_fraction": (missed > self.fallback_threshold).float().mean().detach()}

@@ -134,9 +134,9 @@ def _composite_fwd(sigma: Tensor, sample_rgb: Tensor, t_vals: Tensor, white_bg: 
         raise ValueError(f"unsupported dtype {dt}")
     dev = t_vals.device
     sigma = sigma.contiguous()
-    sample_rgb = sample_rgb.contiguous()
+    sample_rgb = None if sample_rgb is None else sample_rgb.contiguous()
     t_vals = t_vals.contiguous()
-    if sigma.numel() != n * S or sample_rgb.numel() != n * S * 3:
+    if sigma.numel() != n * S or (sample_rgb is not None and sample_rgb.numel() != n * S * 3):
         raise ValueError("sigma must hold N*S values and sample_rgb N*S*3")
     weights = torch.empty((n, S), dtype=dt, device=dev)
     rgb = torch.empty((n, 3), dtype=dt, device=dev)
@@ -153,11 +153,11 @@ def _composite_bwd(sigma, sample_rgb, t_vals, white_bg, g_rgb, g_depth, g_acc, g
     name = "nerf_composite_backward" if dt == torch.float32 else "nerf_composite_backward_f64"
     dev = t_vals.device
     d_sigma = torch.empty((n * S, 1), dtype=dt, device=dev)
-    d_rgb = torch.empty((n * S, 3), dtype=dt, device=dev)
+    d_rgb = None if sample_rgb is None else torch.empty((n * S, 3), dtype=dt, device=dev)
     c = lambda t: None if t is None else t.contiguous()
     if g_rgb is None:
         g_rgb = torch.zeros((n, 3), dtype=dt, device=dev)
-    call(name, n, S, int(bool(white_bg)), ptr(sigma.contiguous()), ptr(sample_rgb.contiguous()),
+    call(name, n, S, int(bool(white_bg)), ptr(sigma.contiguous()), ptr(c(sample_rgb)),
          ptr(t_vals.contiguous()), ptr(c(g_rgb)), ptr(c(g_depth)), ptr(c(g_acc)), ptr(c(g_w)),
          ptr(d_sigma), ptr(d_rgb), stream(dev))
     return d_sigma, d_rgb
@@ -171,15 +171,19 @@ class CompositeFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, sample_rgb, sigma, t_vals, white_bg: bool):
         w, rgb, depth, acc = _composite_fwd(sigma, sample_rgb, t_vals, white_bg)
-        ctx.save_for_backward(sample_rgb, sigma, t_vals)
+        ctx.save_for_backward(sample_rgb if sample_rgb is not None else sigma, sigma, t_vals)
+        ctx.has_rgb = sample_rgb is not None
         ctx.white_bg = white_bg
         return w, rgb, depth, acc
 
     @staticmethod
     def backward(ctx, g_w, g_rgb, g_depth, g_acc):
         sample_rgb, sigma, t_vals = ctx.saved_tensors
+        if not ctx.has_rgb:
+            sample_rgb = None
         d_sigma, d_rgb = _composite_bwd(sigma, sample_rgb, t_vals, ctx.white_bg, g_rgb, g_depth, g_acc, g_w)
-        return d_rgb.reshape(sample_rgb.shape), d_sigma.reshape(sigma.shape), None, None
+        d_rgb = None if d_rgb is None else d_rgb.reshape(sample_rgb.shape)
+        return d_rgb, d_sigma.reshape(sigma.shape), None, None
 
 
 def sigma_to_alpha(sigma: Tensor, diffs: Tensor) -> Tensor:
@@ -188,13 +192,52 @@ def sigma_to_alpha(sigma: Tensor, diffs: Tensor) -> Tensor:
 
 
 def compute_weights(sigma: Tensor, t_vals: Tensor, N_samples: int) -> Tensor:
-    """ray_utils.py:426-482 via the compositing kernel."""
-    n = t_vals.shape[0]
-    dummy_rgb = torch.zeros((n * int(N_samples), 3), dtype=t_vals.dtype, device=t_vals.device)
-    return _composite_fwd(sigma.reshape(-1, 1), dummy_rgb, t_vals, False)[0]
+    """ray_utils.py:426-482 via the compositing kernel (no colours)."""
+    return _composite_fwd(sigma.reshape(-1, 1), None, t_vals, False)[0]
 
 
 def post_process_model_output(sample_rgb: Tensor, sigma: Tensor, t_vals: Tensor, white_bg: bool = False):
     """ray_utils.py:484-551: dict with acc_map (N,), weights (N,S), pred_rgb (N,3), pred_depth (N,)."""
     w, rgb, depth, acc = CompositeFunction.apply(sample_rgb, sigma, t_vals, bool(white_bg))
     return {"acc_map": acc, "weights": w, "pred_rgb": rgb, "pred_depth": depth}
+
+
+class MSEFunction(torch.autograd.Function):
+    """tf.keras.losses.MeanSquaredError (model.py:32) on the GPU: one kernel writes
+    the loss and its gradient 2 (pred - true) / (N * 3)."""
+
+    @staticmethod
+    def forward(ctx, pred: Tensor, target: Tensor):
+        pred = _f32(pred)
+        target = _f32(target)
+        n = pred.shape[0]
+        g = torch.empty_like(pred)
+        loss = torch.empty((), dtype=torch.float32, device=pred.device)
+        call("nerf_mse_grad", n, ptr(pred), ptr(target), 1.0, ptr(g), ptr(loss), stream(pred.device))
+        ctx.save_for_backward(g)
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return g * grad_out, None
+
+
+def mse_loss(y_true: Tensor, y_pred: Tensor) -> Tensor:
+    return MSEFunction.apply(y_pred, y_true)
+
+
+def interlevel(prop_edges: Tensor, prop_weights: Tensor, t_fine: Tensor, fine_weights: Tensor, far: Tensor,
+               want_grad: bool = False, grad_scale: float = 1.0):
+    """Histogram upper-bound loss of the proposal against the fine weights.
+    Returns (missed (N,), loss (N,), d loss / d prop_weights (N, n_prop) * grad_scale or None)."""
+    n, Sp = prop_weights.shape
+    Sf = t_fine.shape[1]
+    dev = prop_weights.device
+    missed = torch.empty((n,), dtype=torch.float32, device=dev)
+    loss = torch.empty((n,), dtype=torch.float32, device=dev)
+    g = torch.empty((n, Sp), dtype=torch.float32, device=dev) if want_grad else None
+    call("nerf_proposal_interlevel", n, Sp, Sf, ptr(_f32(prop_edges)), ptr(_f32(prop_weights.detach())),
+         ptr(_f32(t_fine)), ptr(_f32(fine_weights.detach())), ptr(_f32(far).reshape(n, 1)), ptr(missed),
+         ptr(loss), ptr(g), float(grad_scale), stream(dev))
+    return missed, loss, g

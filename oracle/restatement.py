@@ -83,6 +83,20 @@ def _c(x, like: Tensor) -> Tensor:
     return torch.tensor(x, dtype=like.dtype)
 
 
+def _sqrt(x: Tensor) -> Tensor:
+    """Correctly rounded square root (IEEE, as Eigen/TF's sqrt).  torch's CPU
+    float32 sqrt is not always correctly rounded; the float64 root of a float32
+    value rounds to the correctly rounded float32 root."""
+    if x.dtype == torch.float32:
+        return torch.sqrt(x.double()).float()
+    return torch.sqrt(x)
+
+
+def _pow_f32(base: float, e: float, dtype) -> Tensor:
+    """base ** e evaluated in float64 and rounded once to ``dtype``."""
+    return torch.tensor(float(np.float64(np.float32(base)) ** np.float64(e)), dtype=dtype)
+
+
 # --------------------------------------------------------------------------
 # Rays (ray_utils.py:6-106, pose_utils.py:44-106)
 # --------------------------------------------------------------------------
@@ -125,7 +139,7 @@ def get_rays_tf(H: int, W: int, intrinsic: Tensor, c2w: Tensor):
         cols.append(acc)
     rd = torch.stack(cols, dim=1)
     sq = rd * rd
-    mag = torch.sqrt((sq[:, 0] + sq[:, 1]) + sq[:, 2])  # :100
+    mag = _sqrt((sq[:, 0] + sq[:, 1]) + sq[:, 2])  # :100
     rd = rd / (mag[:, None] + EPS)  # :101
     ro = c2w[:3, 3].expand(rd.shape[0], 3).clone()
     return ro, rd
@@ -402,9 +416,10 @@ def train_step_grads(params, spec, w_coarse, w_fine, rays_o, rays_d, near, far, 
 
 
 def exponential_decay(step: int, initial=5e-4, decay_steps=500000, decay_rate=0.1, dtype=torch.float32) -> Tensor:
-    """model.py:413-417: ExponentialDecay, non-staircase, evaluated in float32."""
+    """model.py:413-417: ExponentialDecay, non-staircase: initial * rate ** (step / decay_steps)
+    with float32 operands; the power is rounded once from float64."""
     p = torch.tensor(float(step), dtype=dtype) / torch.tensor(float(decay_steps), dtype=dtype)
-    return torch.tensor(initial, dtype=dtype) * torch.pow(torch.tensor(decay_rate, dtype=dtype), p)
+    return torch.tensor(initial, dtype=dtype) * _pow_f32(decay_rate, float(p), dtype)
 
 
 class AdamTF:
@@ -427,11 +442,11 @@ class AdamTF:
     def coefficients(self):
         dt = self.b1.dtype
         lr = exponential_decay(self.iterations, dtype=dt)
-        t = torch.tensor(float(self.iterations + 1), dtype=dt)
+        t = float(self.iterations + 1)
         one = torch.tensor(1.0, dtype=dt)
-        b1p = torch.pow(self.b1, t)
-        b2p = torch.pow(self.b2, t)
-        lr_t = lr * torch.sqrt(one - b2p) / (one - b1p)
+        b1p = _pow_f32(float(self.b1), t, dt)
+        b2p = _pow_f32(float(self.b2), t, dt)
+        lr_t = lr * _sqrt(one - b2p) / (one - b1p)
         return lr_t, one - self.b1, one - self.b2
 
     def step(self, variables: List[Tensor], grads: Sequence[Tensor]) -> List[Tensor]:
@@ -440,7 +455,7 @@ class AdamTF:
         for i, (var, g) in enumerate(zip(variables, grads)):
             self.m[i] = self.m[i] + (g - self.m[i]) * omb1
             self.v[i] = self.v[i] + (g * g - self.v[i]) * omb2
-            out.append(var - (self.m[i] * lr_t) / (torch.sqrt(self.v[i]) + self.eps))
+            out.append(var - (self.m[i] * lr_t) / (_sqrt(self.v[i]) + self.eps))
         self.iterations += 1
         return out
 

@@ -1,0 +1,61 @@
+"""CPU checks: the oracle reproduces the committed golden fixture, and the HIP
+library loads, exports every symbol of include/nerf_hip.h and rejects bad
+arguments before launching anything."""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "config0_small.npz")
+
+
+def test_oracle_reproduces_golden(oracle):
+    g = np.load(GOLDEN)
+    spec = oracle.MLPSpec()
+    wc = [torch.from_numpy(w) for w in oracle.glorot_weights(spec, int(g["seed_c"]))]
+    wf = [torch.from_numpy(w) for w in oracle.glorot_weights(spec, int(g["seed_f"]))]
+    T = lambda k: torch.from_numpy(g[k])
+    pc, pf, dC, dF = oracle.forward(oracle.make_params(), spec, wc, wf, T("rays_o"), T("rays_d"), T("near"),
+                                    T("far"), T("u_c"), T("u_f"))
+    assert np.array_equal(dC["t_vals"].numpy(), g["t_coarse"])
+    assert np.array_equal(dC["bin_data"]["bin_edges"].numpy(), g["bin_edges"])
+    assert np.array_equal(dF["piece_idxs"].numpy(), g["piece_idxs"])
+    assert np.array_equal(dF["t_vals"].numpy(), g["t_fine"])
+    for k, ref in (("weights", "w_fine"), ("pred_rgb", "rgb_fine"), ("pred_depth", "depth_fine")):
+        assert np.allclose(pf[k].numpy(), g[ref], rtol=1e-6, atol=1e-7), k
+
+
+def test_library_exports_header_symbols(nerf):
+    lib = nerf._lib.load()
+    names = nerf._lib.header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(nerf._lib._SIGS), "ctypes signatures out of sync with the header"
+    assert lib.nerf_abi_version() == 1
+
+
+def test_size_queries(nerf):
+    lib = nerf._lib.load()
+    d = nerf.MLPConfig().desc()
+    assert lib.nerf_mlp_param_count(ctypes.byref(d)) == 595844
+    assert lib.nerf_mlp_packed_bytes(ctypes.byref(d), 0) > 2 * 595844 * 4 * 0.9
+    assert lib.nerf_mlp_packed_bytes(ctypes.byref(d), 1) < lib.nerf_mlp_packed_bytes(ctypes.byref(d), 0)
+    bad = nerf._lib.desc(depth=8, width=100)
+    assert lib.nerf_mlp_param_count(ctypes.byref(bad)) == -1
+    prop = nerf.MLPConfig(depth=4, width=64, skip=-1, l_xyz=8, l_dir=0).desc()
+    assert lib.nerf_mlp_param_count(ctypes.byref(prop)) == 51 * 64 + 64 + 3 * (64 * 64 + 64) + 64 + 1
+
+
+def test_invalid_arguments_rejected_without_launch(nerf):
+    lib = nerf._lib.load()
+    rc = lib.nerf_get_rays(0, 10, None, None, None, None, None)
+    assert rc == nerf._lib.NERF_ERR_INVALID_ARG
+    assert b"H=0" in lib.nerf_last_error()
+    rc = lib.nerf_create_input_batch_fine(10, 65, 128, *([None] * 12), None)
+    assert rc == nerf._lib.NERF_ERR_INVALID_ARG
+    assert b"n_coarse=65" in lib.nerf_last_error()
+    rc = lib.nerf_composite_forward(10, 1000, 1, *([None] * 7), None)
+    assert rc == nerf._lib.NERF_ERR_INVALID_ARG

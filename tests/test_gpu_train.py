@@ -1,0 +1,146 @@
+"""GPU parity of the hand-written backward (MLP + compositing), the train step and
+Adam against the oracle's autograd gradients.
+
+Gradient bar: per Keras layer, ||g_hip - g_ref||_F <= 1e-3 ||g_ref||_F.  A
+normalised norm rather than a max: a ReLU whose pre-activation is within
+float32 rounding of 0 can take the other branch on the other side, which moves
+one sample's contribution of one unit and nothing else; a real indexing or
+formula error moves the whole layer."""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_parity import DEV, _oracle_weights, _rays_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _split(mlp, flat):
+    out, off = [], 0
+    for _, fi, fo in mlp.config.layer_shapes():
+        out.append(flat[off:off + fi * fo].view(fi, fo))
+        off += fi * fo
+        out.append(flat[off:off + fo])
+        off += fo
+    return out
+
+
+def _check_layers(mlp, g_hip, g_ref, tol=1e-3, what=""):
+    names = [n for n, _, _ in mlp.config.layer_shapes()]
+    for i, (a, b) in enumerate(zip(_split(mlp, g_hip.detach().cpu()), g_ref)):
+        b = b.detach().double().reshape(a.shape)
+        a = a.double()
+        scale = float(b.norm())
+        err = float((a - b).norm())
+        kind = "kernel" if i % 2 == 0 else "bias"
+        assert err <= tol * max(scale, 1e-12), (f"{what} {names[i // 2]} {kind}: |err|_F {err:.3g} |ref|_F "
+                                                f"{scale:.3g} max err {float((a - b).abs().max()):.3g}")
+
+
+@pytest.mark.parametrize("cfg", ["nerf", "small", "proposal"])
+def test_mlp_backward(nerf, oracle, cfg):
+    if cfg == "nerf":
+        conf, spec = nerf.MLPConfig(), oracle.MLPSpec()
+    elif cfg == "small":
+        conf, spec = nerf.MLPConfig(depth=4, width=32, skip=1), oracle.MLPSpec(D=4, W=32, skip=1)
+    else:
+        conf, spec = nerf.MLPConfig(depth=4, width=64, skip=-1, l_xyz=8, l_dir=0), None
+    mlp = nerf.NerfMLP("coarse", conf, "fp32", seed=5)
+    with torch.no_grad():
+        for _, k, b in mlp.layer_slices():
+            b.copy_(0.05 * torch.randn_like(b))
+    mlp.repack()
+    M = 4099
+    g = torch.Generator().manual_seed(2)
+    xyz = (torch.rand((M, 3), generator=g) * 6 - 3).float()
+    dirs = torch.nn.functional.normalize(torch.randn((M, 3), generator=g), dim=1).float()
+    g_rgb = torch.randn((M, 3), generator=g).float()
+    g_sig = torch.randn((M, 1), generator=g).float()
+    mlp.params.requires_grad_(True)
+    mlp.params.grad = None
+    if conf.l_dir:
+        rgb, sigma = mlp((xyz.to(DEV), dirs.to(DEV)))
+        L = (rgb * g_rgb.to(DEV)).sum() + (sigma * g_sig.to(DEV)).sum()
+    else:
+        sigma = mlp((xyz.to(DEV), dirs.to(DEV)))
+        L = (sigma * g_sig.to(DEV)).sum()
+    L.backward()
+    W = [w.clone().requires_grad_(True) for w in _oracle_weights(mlp)]
+    if conf.l_dir:
+        rgb_r, sig_r = oracle.mlp_forward(spec, W, xyz, dirs)
+        Lr = (rgb_r * g_rgb).sum() + (sig_r * g_sig).sum()
+    else:
+        sig_r = _proposal_forward(oracle, W, xyz)
+        Lr = (sig_r * g_sig).sum()
+    ref = torch.autograd.grad(Lr, W)
+    _check_layers(mlp, mlp.params.grad, ref, what=cfg)
+
+
+def _proposal_forward(oracle, W, xyz):
+    """Density-only MLP: 4 ReLU layers on the L=8 encoding, softplus density."""
+    h = oracle.positional_encoding(xyz, 8)
+    for i in range(4):
+        h = torch.relu(h @ W[2 * i] + W[2 * i + 1])
+    return torch.nn.functional.softplus(h @ W[8] + W[9])
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_train_step_gradients(nerf, oracle, n):
+    """Config 0 (1024 rays) and full width at 4096 rays x (64 + 128) samples.
+    The coarse gradients are compared end to end; the fine gradients on the
+    same fine sample positions (see test_nerf_forward_config0)."""
+    ro, rd, near, far = _rays_batch(nerf, n, seed=21)
+    net = nerf.NeRF(nerf.Params(), "fp32", seed=3)
+    rgb = torch.rand((n, 3), device=DEV)
+    u_c = torch.rand((n, 64), device=DEV)
+    u_f = torch.rand((n, 128), device=DEV)
+    for mdl in (net.coarse_model, net.fine_model):
+        mdl.params.requires_grad_(True)
+        mdl.params.grad = None
+    pc, pf = net.forward(ro, rd, near, far, u_c, u_f, return_indices=True)
+    (nerf.ray_utils.mse_loss(rgb, pc["pred_rgb"]) + nerf.ray_utils.mse_loss(rgb, pf["pred_rgb"])).backward()
+    spec = oracle.MLPSpec()
+    Po = oracle.make_params()
+    Wc = [w.requires_grad_(True) for w in _oracle_weights(net.coarse_model)]
+    dC = oracle.create_input_batch_coarse_model(Po, ro.cpu(), rd.cpu(), near.cpu(), far.cpu(), u_vals=u_c.cpu())
+    rgb_c, sig_c = oracle.mlp_forward(spec, Wc, dC["xyz_inputs"], dC["dir_inputs"])
+    ppc = oracle.post_process_model_output(rgb_c, sig_c, dC["t_vals"], True)
+    gc = torch.autograd.grad(oracle.mse_loss(rgb.cpu(), ppc["pred_rgb"]), Wc)
+    _check_layers(net.coarse_model, net.coarse_model.params.grad, gc, what="coarse")
+    Wf = [w.requires_grad_(True) for w in _oracle_weights(net.fine_model)]
+    t_f = pf["t_vals"].detach().cpu()
+    xyz = (ro.cpu()[:, None, :] + t_f[..., None] * rd.cpu()[:, None, :]).reshape(-1, 3)
+    dirs = rd.cpu()[:, None, :].expand(n, t_f.shape[1], 3).reshape(-1, 3)
+    rgb_f, sig_f = oracle.mlp_forward(spec, Wf, xyz, dirs)
+    ppf = oracle.post_process_model_output(rgb_f, sig_f, t_f, True)
+    gf = torch.autograd.grad(oracle.mse_loss(rgb.cpu(), ppf["pred_rgb"]), Wf)
+    _check_layers(net.fine_model, net.fine_model.params.grad, gf, what="fine")
+
+
+def test_adam_step_matches_oracle(nerf, oracle):
+    mlp = nerf.NerfMLP("coarse", nerf.MLPConfig(depth=4, width=32, skip=1), "fp32", seed=1)
+    opt = nerf.AdamTF([mlp])
+    p0 = mlp.params.detach().cpu().clone()
+    ref_opt = oracle.AdamTF([p0])
+    var = [p0.clone()]
+    for step in range(3):
+        g = torch.randn(mlp.n_params) * (10.0 ** (step - 2))
+        opt.apply_gradients([g.to(DEV)])
+        var = ref_opt.step(var, [g])
+    torch.cuda.synchronize()
+    got = mlp.params.detach().cpu()
+    # same float32 operations in the same order: bit-exact
+    assert torch.equal(got, var[0]), float((got - var[0]).abs().max())
+
+
+def test_train_step_runs_and_learns(nerf):
+    """A few steps on one fixed batch of the blob scene: the loss goes down."""
+    sc = nerf.scene
+    scene = sc.BlobScene.from_seed(0)
+    ro, rd, near, far = _rays_batch(nerf, 2048, seed=8)
+    gt, _, _ = scene.render(ro.double(), rd.double(), n_samples=256)
+    gt = gt.float()
+    net = nerf.NeRF(nerf.Params(), "fp32", seed=0)
+    losses = [float(net.train_step(ro, rd, near, far, gt)["fine_loss"]) for _ in range(30)]
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.5 * losses[0], losses
