@@ -38,9 +38,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--train-steps", type=int, default=10)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--forms", default="plain_fp32,plain_bf16,learned_bf16",
+                    help="comma list; the first is the headline `value`")
+    ap.add_argument("--learned-warmup-steps", type=int, default=300,
+                    help="train steps of the learned form before its render is timed")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -60,7 +65,13 @@ def main():
     sc = nerf.scene
     scene = sc.BlobScene.from_seed(0)
     P = nerf.Params()
-    net = nerf.NeRF(P, args.precision, seed=0, device=dev)
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).uniform_()
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---------------- inputs resident in HBM
     H = W = 800
@@ -73,7 +84,6 @@ def main():
     chunk = 32768
 
     # training batches: 4096 random rays across 100 seed-placed 800x800 views
-    Kt = sc.intrinsics(H, W, 1111.1)
     poses = sc.random_poses(100, 7)
     rng = np.random.default_rng(0)
     batches = []
@@ -82,7 +92,7 @@ def main():
         pix = rng.integers(0, H * W, size=4096)
         o_l, d_l = [], []
         for v in np.unique(views):
-            o, d = nerf.ray_utils.get_rays_tf(H, W, Kt.astype(np.float32), poses[v].astype(np.float32), device=dev)
+            o, d = nerf.ray_utils.get_rays_tf(H, W, K.astype(np.float32), poses[v].astype(np.float32), device=dev)
             sel = torch.from_numpy(pix[views == v]).to(dev)
             o_l.append(o[sel])
             d_l.append(d[sel])
@@ -93,68 +103,112 @@ def main():
                         torch.full((4096, 1), sc.FAR, device=dev), gt.float().contiguous()))
     torch.cuda.synchronize()
 
-    # ---------------- render: warmup + timed steps
+    forms = [f for f in args.forms.split(",") if f]
+    results = {}
+    for form in forms:
+        results[form] = measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches)
+    head = results[forms[0]]
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    prec = "fp32" if forms[0].endswith("fp32") else "bf16"
+    peak = FP32_PEAK_TFLOPS if prec == "fp32" else BF16_PEAK_TFLOPS
+    line = {
+        "metric": METRIC, "value": head["render_rays_per_s"], "unit": "rays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["render_ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": prec, "data": "synthetic (procedural Gaussian-blob scene, random-init weights)",
+        "config": {"workload": f"{forms[0]}: coarse+fine NeRF render of one 800x800 view (640,000 rays, "
+                               "32,768-ray chunks, 64+128 samples, 8x256 MLPs); train on 4096-ray batches",
+                   "rays_per_step": n_rays, "chunk": chunk, "train_batch": 4096},
+        "train_rays_per_s": head["train_rays_per_s"],
+        "train_ms_per_step": head["train_ms_per_step"],
+        "render_tflops": round(head["render_rays_per_s"] * RENDER_FLOP_PER_RAY / 1e12, 2),
+        "roofline": {"bound": "mfma", "kernel": "mlp_fwd (fine model, 32,768 rays x 192 samples per launch)",
+                     "achieved": head["mlp_tflops"], "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(head["mlp_tflops"] / peak, 4), "traffic": None,
+                     "avg_launch_ms": head["mlp_avg_launch_ms"], "flop_per_launch": head["mlp_flop_per_launch"]},
+        "cpu_baseline": cpu,
+        "forms": results,
+    }
+    if "learned_bf16" in results and "plain_fp32" in results and results["plain_fp32"]["train_rays_per_s"]:
+        line["learned_vs_plain"] = {
+            "render_speedup": round(results["learned_bf16"]["render_rays_per_s"] /
+                                    results["plain_fp32"]["render_rays_per_s"], 2),
+            "train_speedup": round(results["learned_bf16"]["train_rays_per_s"] /
+                                   results["plain_fp32"]["train_rays_per_s"], 2),
+            "psnr": "see nerf-tf2_amd/quality.py (2000-step training runs)"}
+    print(json.dumps(line))
+
+
+def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
+    """Render (timed, K steps of one 800x800 view) and train (timed) rays/s of one form."""
+    import numpy as np
+    import torch
+    n_rays = ro.shape[0]
+    learned = form.startswith("learned")
+    prec = "bf16" if form.endswith("bf16") else "fp32"
+    if learned:
+        net = nerf.ProposalNeRF(P, prec, seed=0, device=dev)
+        for i in range(args.learned_warmup_steps):   # let the proposal learn before timing its render
+            net.train_step(*batches[i % 4])
+        torch.cuda.synchronize()
+    else:
+        net = nerf.NeRF(P, prec, seed=0, device=dev)
     mlp_ms = []
 
-    def render(timed_kernel: bool):
-        out = []
+    def render(timed_kernel):
+        fb = 0.0
         for i in range(0, n_rays, chunk):
             sl = slice(i, min(i + chunk, n_rays))
-            out.append(_render_chunk(net, nerf, ro[sl], rd[sl], near[sl], far[sl], timed_kernel, mlp_ms))
-        return out
+            if learned:
+                with torch.no_grad():
+                    net.forward(ro[sl], rd[sl], near[sl], far[sl], fallback=True)
+                fb += net.last_fallback_fraction * (sl.stop - sl.start)
+            else:
+                _render_chunk(net, nerf, ro[sl], rd[sl], near[sl], far[sl], timed_kernel, mlp_ms)
+        return fb / n_rays
 
     for _ in range(args.warmup):
         render(False)
     torch.cuda.synchronize()
     mlp_ms.clear()
     t0 = time.perf_counter()
+    fb = 0.0
     for _ in range(args.steps):
-        render(True)
+        fb = render(True)
     torch.cuda.synchronize()
     t_render = time.perf_counter() - t0
-    render_rays_s = n_rays * args.steps / t_render
-
-    # ---------------- train
-    for i in range(2):
-        net.train_step(*batches[i % 4])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.train_steps):
-        net.train_step(*batches[i % 4])
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    train_rays_s = 4096 * args.train_steps / t_train
-
-    # ---------------- roofline of the dominant kernel (fine MLP forward, full chunks)
+    out = {"render_rays_per_s": round(n_rays * args.steps / t_render, 1),
+           "render_ms_per_step": round(1e3 * t_render / args.steps, 3),
+           "train_rays_per_s": None, "train_ms_per_step": None}
+    if args.train_steps > 0:
+        for i in range(2):
+            net.train_step(*batches[i % 4])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.train_steps):
+            net.train_step(*batches[i % 4])
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        out["train_rays_per_s"] = round(4096 * args.train_steps / t_train, 1)
+        out["train_ms_per_step"] = round(1e3 * t_train / args.train_steps, 3)
+    if learned:
+        out["fallback_fraction"] = round(fb, 4)
+        out["warmup_train_steps"] = args.learned_warmup_steps
     full = [ms for ms, m in mlp_ms if m == chunk * 192]
-    avg_ms = float(np.mean(full)) if full else float("nan")
-    flop = chunk * 192 * MLP_FLOP_PER_SAMPLE
-    achieved = flop / (avg_ms * 1e-3) / 1e12
-    peak = FP32_PEAK_TFLOPS if args.precision == "fp32" else BF16_PEAK_TFLOPS
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_seconds)
-
-    line = {
-        "metric": METRIC, "value": round(render_rays_s, 1), "unit": "rays/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_render / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": args.precision, "data": "synthetic (procedural Gaussian-blob scene, random-init weights)",
-        "config": {"workload": "plain coarse+fine NeRF render of one 800x800 view (640,000 rays, 32,768-ray "
-                               "chunks, 64+128 samples, 8x256 MLPs); train on 4096-ray batches",
-                   "rays_per_step": n_rays, "chunk": chunk, "train_batch": 4096},
-        "train_rays_per_s": round(train_rays_s, 1),
-        "train_ms_per_step": round(1e3 * t_train / args.train_steps, 3),
-        "render_tflops": round(render_rays_s * RENDER_FLOP_PER_RAY / 1e12, 2),
-        "roofline": {"bound": "mfma", "kernel": "mlp_fwd (fine model, 32,768 rays x 192 samples per launch)",
-                     "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "avg_launch_ms": round(avg_ms, 3), "flop_per_launch": flop},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line))
+    if full:
+        avg_ms = float(np.mean(full))
+        flop = chunk * 192 * MLP_FLOP_PER_SAMPLE
+        out["mlp_avg_launch_ms"] = round(avg_ms, 3)
+        out["mlp_flop_per_launch"] = flop
+        out["mlp_tflops"] = round(flop / (avg_ms * 1e-3) / 1e12, 2)
+    del net
+    torch.cuda.empty_cache()
+    return out
 
 
 def _render_chunk(net, nerf, ro, rd, near, far, timed_kernel, mlp_ms):

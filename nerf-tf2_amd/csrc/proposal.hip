@@ -5,7 +5,8 @@
 //
 // For fine interval i = [t_i, t_{i+1}] (the last one ends at `far`), the
 // proposal's outer measure is bound_i = sum of w_p over the proposal bins that
-// overlap it.  missed_i = max(0, w_i - bound_i); the ray's missed mass is
+// overlap it, summed directly over the bin range (a difference of two float32
+// cdf values would cancel for the many tiny fine weights).  missed_i = max(0, w_i - bound_i); the ray's missed mass is
 // sum_i missed_i and its loss sum_i missed_i^2 / (w_i + 1e-7), with w_i held
 // constant.  d loss / d w_p[k] = sum over intervals i whose bin range holds k of
 // -2 missed_i / (w_i + 1e-7), summed by one lane per bin in a fixed order.
@@ -20,22 +21,14 @@ __global__ __launch_bounds__(256) void interlevel_kernel(
     int64_t n_rays, int Sp, int Sf, const float* __restrict__ edges, const float* __restrict__ wp,
     const float* __restrict__ tf, const float* __restrict__ wf, const float* __restrict__ far,
     float* __restrict__ missed, float* __restrict__ loss, float* __restrict__ g_wp, float gscale) {
-  __shared__ float s_e[kW][kMaxSp + 1], s_c[kW][kMaxSp + 1], s_coef[kW][kMaxSf];
+  __shared__ float s_e[kW][kMaxSp + 1], s_w[kW][kMaxSp], s_coef[kW][kMaxSf];
   __shared__ int s_lo[kW][kMaxSf], s_hi[kW][kMaxSf];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t ray = (int64_t)blockIdx.x * kW + w;
   if (ray >= n_rays) return;
-  // proposal edges and the exclusive cdf of its weights (wave scan)
-  const float wv = lane < Sp ? wp[ray * Sp + lane] : 0.f;
-  float incl = wv;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const float o = shfl_up(incl, d);
-    if (lane >= d) incl += o;
-  }
+  // proposal edges and weights
   if (lane <= Sp) s_e[w][lane] = edges[ray * (Sp + 1) + lane];
-  if (lane < Sp) s_c[w][lane + 1] = incl;
-  if (lane == 0) s_c[w][0] = 0.f;
+  if (lane < Sp) s_w[w][lane] = wp[ray * Sp + lane];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -59,7 +52,8 @@ __global__ __launch_bounds__(256) void interlevel_kernel(
     }
     int khi = a;
     khi = khi > Sp ? Sp : (khi < klo + 1 ? klo + 1 : khi);
-    const float bound = s_c[w][khi] - s_c[w][klo];
+    float bound = 0.f;
+    for (int k = klo; k < khi; ++k) bound += s_w[w][k];
     const float wi = wf[ray * Sf + i];
     const float mi = fmaxf(wi - bound, 0.f);
     ms += mi;

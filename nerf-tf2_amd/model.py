@@ -64,21 +64,35 @@ PRECISIONS = {"fp32": _lib.PREC_F32, "bf16": _lib.PREC_BF16}
 
 class MLPFunction(torch.autograd.Function):
     """Fused encoder+MLP forward/backward (HIP) as an autograd Function.
-    Inputs: flat params (differentiable), then the sample description."""
+    Inputs: flat params (differentiable), then the sample description.
+
+    Without recomputation the forward writes the training stash (every layer's
+    input) for all M samples.  With ``mlp.recompute_chunk`` set (ray input only)
+    the forward keeps just its inputs; the backward re-runs the forward one
+    chunk of rays at a time to rebuild that chunk's stash, runs the dX chain on
+    it and accumulates the weight gradients, so peak memory scales with the
+    chunk, not the batch."""
 
     @staticmethod
     def forward(ctx, params: Tensor, mlp: "NerfMLP", xyz, dirs, rays_o, rays_d, t_vals):
         need_grad = ctx.needs_input_grad[0]
         M = mlp._num_samples(xyz, t_vals)
         dev = params.device
-        stash = mlp._alloc_stash(M) if need_grad else None
+        recompute = need_grad and mlp.recompute_chunk is not None and t_vals is not None and \
+            M > mlp.recompute_chunk * t_vals.shape[1]
+        stash = mlp._alloc_stash(M) if (need_grad and not recompute) else None
         rgb = torch.empty((M, 3), dtype=torch.float32, device=dev) if mlp.config.l_dir else None
         sigma = torch.empty((M, 1), dtype=torch.float32, device=dev)
         mlp._launch_forward(M, xyz, dirs, rays_o, rays_d, t_vals, rgb, sigma, stash)
         ctx.mlp = mlp
         ctx.M = M
+        ctx.recompute = recompute
         if need_grad:
-            ctx.save_for_backward(rgb if rgb is not None else sigma, sigma, stash)
+            rgb_s = rgb if rgb is not None else sigma
+            if recompute:
+                ctx.save_for_backward(rgb_s, sigma, rays_o, rays_d, t_vals)
+            else:
+                ctx.save_for_backward(rgb_s, sigma, stash)
         if rgb is None:
             return sigma
         return rgb, sigma
@@ -90,10 +104,16 @@ class MLPFunction(torch.autograd.Function):
             g_rgb, g_sigma = grads
         else:
             g_rgb, g_sigma = None, grads[0]
-        rgb, sigma, stash = ctx.saved_tensors
-        if not mlp.config.l_dir:
-            rgb = None
-        grad = mlp.backward_to_params(ctx.M, g_rgb, g_sigma, rgb, sigma, stash)
+        if ctx.recompute:
+            rgb, sigma, ro, rd, tv = ctx.saved_tensors
+            if not mlp.config.l_dir:
+                rgb = None
+            grad = mlp.backward_recompute(g_rgb, g_sigma, rgb, sigma, ro, rd, tv)
+        else:
+            rgb, sigma, stash = ctx.saved_tensors
+            if not mlp.config.l_dir:
+                rgb = None
+            grad = mlp.backward_to_params(ctx.M, g_rgb, g_sigma, rgb, sigma, stash)
         return grad, None, None, None, None, None, None
 
 
@@ -118,6 +138,7 @@ class NerfMLP:
                                   dtype=torch.uint8, device=self.device)
         self.stash_per_sample = int(lib.nerf_mlp_stash_floats_per_sample(ctypes_ref(self._desc)))
         self.dz_per_sample = int(lib.nerf_mlp_dz_floats_per_sample(ctypes_ref(self._desc)))
+        self.recompute_chunk: Optional[int] = None  # rays per recomputation chunk (None: stash all)
         self.repack()
 
     # -- parameters --------------------------------------------------------
@@ -187,7 +208,7 @@ class NerfMLP:
                                  t_vals.contiguous())
 
     # -- backward ----------------------------------------------------------
-    def backward_to_params(self, M, g_rgb, g_sigma, rgb, sigma, stash) -> Tensor:
+    def _backward_into(self, M, g_rgb, g_sigma, rgb, sigma, stash, grad, accumulate, work=None):
         dz = torch.empty((M, self.dz_per_sample), dtype=torch.float32, device=self.device)
         c = lambda t: None if t is None else t.contiguous()
         if g_sigma is None:
@@ -196,11 +217,35 @@ class NerfMLP:
             g_rgb = torch.zeros((M, 3), dtype=torch.float32, device=self.device)
         call("nerf_mlp_backward", ctypes_ref(self._desc), self.prec, ptr(self.packed), M, ptr(c(g_rgb)),
              ptr(c(g_sigma)), ptr(c(rgb)), ptr(c(sigma)), ptr(stash), ptr(dz), stream(self.device))
-        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
-        ws = int(_lib.load().nerf_mlp_wgrad_workspace_bytes(ctypes_ref(self._desc), M))
-        work = torch.empty(max(ws, 1), dtype=torch.uint8, device=self.device)
+        if work is None:
+            ws = int(_lib.load().nerf_mlp_wgrad_workspace_bytes(ctypes_ref(self._desc), M))
+            work = torch.empty(max(ws, 1), dtype=torch.uint8, device=self.device)
         call("nerf_mlp_weight_grads", ctypes_ref(self._desc), self.prec, M, ptr(stash), ptr(dz), ptr(grad),
-             0, ptr(work), stream(self.device))
+             int(accumulate), ptr(work), stream(self.device))
+
+    def backward_to_params(self, M, g_rgb, g_sigma, rgb, sigma, stash) -> Tensor:
+        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+        self._backward_into(M, g_rgb, g_sigma, rgb, sigma, stash, grad, False)
+        return grad
+
+    def backward_recompute(self, g_rgb, g_sigma, rgb, sigma, rays_o, rays_d, t_vals) -> Tensor:
+        n, S = t_vals.shape
+        R = int(self.recompute_chunk)
+        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+        ws = int(_lib.load().nerf_mlp_wgrad_workspace_bytes(ctypes_ref(self._desc), R * S))
+        work = torch.empty(max(ws, 1), dtype=torch.uint8, device=self.device)
+        stash = self._alloc_stash(R * S)
+        for r0 in range(0, n, R):
+            r1 = min(n, r0 + R)
+            a, b = r0 * S, r1 * S
+            st = stash[: b - a]
+            scratch_rgb = torch.empty((b - a, 3), dtype=torch.float32, device=self.device) \
+                if self.config.l_dir else None
+            scratch_sig = torch.empty((b - a, 1), dtype=torch.float32, device=self.device)
+            self._launch_forward(b - a, None, None, rays_o[r0:r1], rays_d[r0:r1], t_vals[r0:r1], scratch_rgb,
+                                 scratch_sig, st)
+            sl = lambda t: None if t is None else t[a:b]
+            self._backward_into(b - a, sl(g_rgb), sl(g_sigma), sl(rgb), sl(sigma), st, grad, r0 > 0, work)
         return grad
 
 

@@ -1,0 +1,142 @@
+"""Quality and throughput of the three forms on the Gaussian-blob scene.
+
+    python -m nerf-tf2_amd.quality --steps 2000          (run from the repo root)
+
+Trains, on identical 4096-ray batches drawn across 100 seed-placed 800x800
+views, (a) the plain FP32 coarse+fine NeRF, (b) the same in BF16 (precision
+form, config 2) and (c) the BF16 learned-proposal form (config 4); then
+renders held-out 800x800 views with each and reports per-element PSNR and max
+|rgb error| against the float64 ground truth, the errors of (b) and (c)
+against (a), and render / train rays/s.  Prints one JSON object.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import math
+import sys
+import time
+
+import numpy as np
+import torch
+
+
+def _pkg():
+    return importlib.import_module("nerf-tf2_amd")
+
+
+def make_batches(nerf, scene, n_steps, batch=4096, n_views=100, H=800, W=800, focal=1111.1, seed=7,
+                 device="cuda"):
+    """n_steps batches of `batch` rays drawn uniformly over (view, pixel), with
+    float64 ground truth (1024-sample closed-form optical depth)."""
+    sc = nerf.scene
+    K = sc.intrinsics(H, W, focal).astype(np.float32)
+    poses = sc.random_poses(n_views, seed)
+    ro_all, rd_all = [], []
+    for v in range(n_views):
+        o, d = nerf.ray_utils.get_rays_tf(H, W, K, poses[v].astype(np.float32), device=device)
+        ro_all.append(o[:1].clone())        # one origin per view
+        rd_all.append(d)
+    rd_all = torch.stack(rd_all)            # (V, H*W, 3)
+    ro_all = torch.cat(ro_all)              # (V, 3)
+    g = torch.Generator(device=device).manual_seed(seed)
+    out = []
+    for _ in range(n_steps):
+        v = torch.randint(0, n_views, (batch,), device=device, generator=g)
+        p = torch.randint(0, H * W, (batch,), device=device, generator=g)
+        o = ro_all[v].contiguous()
+        d = rd_all[v, p].contiguous()
+        gt, _, _ = scene.render(o.double(), d.double(), n_samples=1024, chunk=4096)
+        out.append((o, d, gt.float().contiguous()))
+    return out
+
+
+def psnr(a, b):
+    mse = float(torch.mean((a.double() - b.double()) ** 2))
+    return -10.0 * math.log10(max(mse, 1e-30))
+
+
+def run(steps=2000, eval_views=2, seed=0, device="cuda"):
+    nerf = _pkg()
+    sc = nerf.scene
+    scene = sc.BlobScene.from_seed(0)
+    P = nerf.Params()
+    t0 = time.perf_counter()
+    batches = make_batches(nerf, scene, steps, device=device)
+    near = torch.full((4096, 1), sc.NEAR, device=device)
+    far = torch.full((4096, 1), sc.FAR, device=device)
+    t_data = time.perf_counter() - t0
+    # held-out views
+    H = W = 800
+    K = sc.intrinsics(H, W, 1111.1)
+    views = []
+    for pose in sc.random_poses(eval_views, 999):
+        ro, rd = nerf.ray_utils.get_rays(H, W, K, pose, device=device)
+        gt, _, _ = scene.render(ro.double(), rd.double(), n_samples=1024, chunk=4096)
+        views.append((ro, rd, gt))
+    forms = {
+        "plain_fp32": lambda: nerf.NeRF(P, "fp32", seed=seed, device=device),
+        "plain_bf16": lambda: nerf.NeRF(P, "bf16", seed=seed, device=device),
+        "learned_bf16": lambda: nerf.ProposalNeRF(P, "bf16", seed=seed, device=device),
+    }
+    res = {"steps": steps, "batch": 4096, "data_seconds": round(t_data, 1)}
+    renders = {}
+    for name, make in forms.items():
+        net = make()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            o, d, gt = batches[i]
+            net.train_step(o, d, near, far, gt)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        preds, fb = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for ro, rd, _ in views:
+            n = ro.shape[0]
+            nr = torch.full((n, 1), sc.NEAR, device=device)
+            fr = torch.full((n, 1), sc.FAR, device=device)
+            preds.append(net.predict(ro, rd, nr, fr)["pred_rgb"])
+            fb.append(getattr(net, "last_fallback_fraction", 0.0))
+        torch.cuda.synchronize()
+        t_render = time.perf_counter() - t0
+        renders[name] = preds
+        gts = [v[2] for v in views]
+        res[name] = {
+            "train_rays_per_s": round(4096 * steps / t_train, 1),
+            "render_rays_per_s": round(sum(v[0].shape[0] for v in views) / t_render, 1),
+            "psnr_vs_fp64_gt": [round(psnr(p, g), 3) for p, g in zip(preds, gts)],
+            "max_abs_rgb_err_vs_fp64_gt": [round(float((p.double() - g).abs().max()), 4) for p, g in zip(preds, gts)],
+        }
+        if fb and name == "learned_bf16":
+            res[name]["fallback_fraction"] = [round(x, 4) for x in fb]
+        if getattr(net, "scaler", None) is not None:
+            res[name]["loss_scale"] = net.scaler.scale
+            res[name]["skipped_steps"] = net.scaler.skipped
+        del net
+        torch.cuda.empty_cache()
+    for name in ("plain_bf16", "learned_bf16"):
+        res[name]["psnr_vs_plain_fp32"] = [round(psnr(a, b), 3) for a, b in zip(renders[name], renders["plain_fp32"])]
+        res[name]["max_abs_rgb_err_vs_plain_fp32"] = [round(float((a - b).abs().max()), 4)
+                                                      for a, b in zip(renders[name], renders["plain_fp32"])]
+        res[name]["psnr_delta_db"] = round(float(np.mean(res[name]["psnr_vs_fp64_gt"]) -
+                                                 np.mean(res["plain_fp32"]["psnr_vs_fp64_gt"])), 3)
+        res[name]["render_speedup_vs_plain_fp32"] = round(res[name]["render_rays_per_s"] /
+                                                         res["plain_fp32"]["render_rays_per_s"], 2)
+        res[name]["train_speedup_vs_plain_fp32"] = round(res[name]["train_rays_per_s"] /
+                                                        res["plain_fp32"]["train_rays_per_s"], 2)
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--eval-views", type=int, default=2)
+    args = ap.parse_args(argv)
+    print(json.dumps(run(args.steps, args.eval_views)))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

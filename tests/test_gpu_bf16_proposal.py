@@ -15,7 +15,61 @@ from test_gpu_train import _split
 pytestmark = pytest.mark.gpu
 
 
+def _r16(x):
+    return x.bfloat16().float()
+
+
+class _Bf16Linear(torch.autograd.Function):
+    """The bf16 kernels' matmul: y = r(x) @ r(W) accumulated in fp32; the
+    backward dX = r(dy) @ r(W)^T (dz is the bf16 B operand) and dW = r(x)^T @ dy
+    (the weight GEMM reads the bf16 stash and the fp32 dz rows)."""
+
+    @staticmethod
+    def forward(ctx, x, w):
+        xr, wr = _r16(x), _r16(w)
+        ctx.save_for_backward(xr, wr)
+        return xr @ wr
+
+    @staticmethod
+    def backward(ctx, dy):
+        xr, wr = ctx.saved_tensors
+        return _r16(dy) @ wr.T, xr.T @ dy
+
+
+class _RoundSTE(torch.autograd.Function):
+    """bf16 rounding of the activations the fp32 heads read; identity backward."""
+
+    @staticmethod
+    def forward(ctx, x):
+        return _r16(x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return g
+
+
+def _bf16_emulated_mlp(oracle, W, xyz, dirs):
+    """model.py:355-392 with the bf16 kernels' rounding points (fp32 biases,
+    activation functions and sigma / rgb heads)."""
+    ex = oracle.positional_encoding(xyz, 10)
+    ed = oracle.positional_encoding(dirs, 4)
+    lin = _Bf16Linear.apply
+    h = ex
+    for i in range(8):
+        h = torch.relu(lin(h, W[2 * i]) + W[2 * i + 1])
+        if i == 4:
+            h = torch.cat([h, ex], -1)
+    sigma = torch.relu(_RoundSTE.apply(h) @ W[16] + W[17])
+    bott = lin(h, W[18]) + W[19]
+    h9 = torch.relu(lin(torch.cat([bott, ed], -1), W[20]) + W[21])
+    rgb = torch.sigmoid(_RoundSTE.apply(h9) @ W[22] + W[23])
+    return rgb, sigma
+
+
 def test_bf16_mlp_forward_and_backward(nerf, oracle):
+    """Against an oracle that rounds exactly where the bf16 kernels round; what is
+    left is the fp32 summation order and ulp-level sin/cos differences that can
+    move an encoding value to the neighbouring bf16."""
     mlp = nerf.NerfMLP("fine", nerf.MLPConfig(), "bf16", seed=4)
     M = 4099
     g = torch.Generator().manual_seed(1)
@@ -27,13 +81,16 @@ def test_bf16_mlp_forward_and_backward(nerf, oracle):
     rgb, sigma = mlp((xyz.to(DEV), dirs.to(DEV)))
     ((rgb * g_rgb.to(DEV)).sum() + (sigma * g_sig.to(DEV)).sum()).backward()
     W = [w.clone().requires_grad_(True) for w in _oracle_weights(mlp)]
-    rgb_r, sig_r = oracle.mlp_forward(oracle.MLPSpec(), W, xyz, dirs)
-    assert float((rgb.detach().cpu() - rgb_r).abs().max()) < 3e-2
-    assert float((sigma.detach().cpu() - sig_r).abs().max()) < 3e-2 * max(1.0, float(sig_r.abs().max()))
-    ref = torch.autograd.grad((rgb_r * g_rgb).sum() + (sig_r * g_sig).sum(), W)
+    rgb_e, sig_e = _bf16_emulated_mlp(oracle, W, xyz, dirs)
+    assert float((rgb.detach().cpu() - rgb_e).abs().max()) < 1e-2
+    assert float((sigma.detach().cpu() - sig_e).abs().max()) < 1e-2 * max(1.0, float(sig_e.abs().max()))
+    ref = torch.autograd.grad((rgb_e * g_rgb).sum() + (sig_e * g_sig).sum(), W)
     for i, (a, b) in enumerate(zip(_split(mlp, mlp.params.grad.detach().cpu()), ref)):
         rel = float((a.double() - b.double()).norm() / (b.double().norm() + 1e-30))
-        assert rel < 5e-2, (i, rel)
+        assert rel < 2e-2, (i, rel)
+    # and against plain float32: the bf16 forward error is small in absolute terms
+    rgb_r, sig_r = oracle.mlp_forward(oracle.MLPSpec(), [w.detach() for w in W], xyz, dirs)
+    assert float((rgb.detach().cpu() - rgb_r).abs().max()) < 3e-2
 
 
 def _interlevel_reference(edges, wp, tf, wf, far):
@@ -79,7 +136,9 @@ def test_interlevel_kernel(nerf):
     g_ref, = torch.autograd.grad(l_ref.sum(), wp64)
     assert torch.allclose(missed.cpu().double(), m_ref.detach(), rtol=1e-4, atol=1e-6)
     assert torch.allclose(loss.cpu().double(), l_ref.detach(), rtol=1e-3, atol=1e-6)
-    assert torch.allclose(gw.cpu().double(), g_ref, rtol=1e-3, atol=1e-5 * float(g_ref.abs().max()))
+    err = (gw.cpu().double() - g_ref).abs()
+    tol = 1e-3 * g_ref.abs() + 1e-5 * float(g_ref.abs().max())
+    assert bool((err <= tol).all()), (float(err.max()), int(err.argmax()))
     # a proposal that puts all its mass where the fine weights are misses nothing
     m2, _, _ = nerf.ray_utils.interlevel(edges.to(DEV), torch.ones(n, Sp, device=DEV), tf.to(DEV), wf.to(DEV),
                                          far.to(DEV))

@@ -144,3 +144,30 @@ def test_train_step_runs_and_learns(nerf):
     losses = [float(net.train_step(ro, rd, near, far, gt)["fine_loss"]) for _ in range(30)]
     assert np.isfinite(losses).all()
     assert losses[-1] < 0.5 * losses[0], losses
+
+
+def test_recompute_matches_stash(nerf):
+    """Chunked activation recomputation gives the same gradients as stashing
+    every activation (only the split-K summation order differs)."""
+    ro, rd, near, far = _rays_batch(nerf, 1536, seed=13)
+    rgb = torch.rand((1536, 3), device=DEV)
+    u_c = torch.rand((1536, 64), device=DEV)
+    u_f = torch.rand((1536, 128), device=DEV)
+    grads = []
+    peaks = []
+    for chunk in (None, 512):
+        net = nerf.NeRF(nerf.Params(), "fp32", seed=4)
+        for mdl in (net.coarse_model, net.fine_model):
+            mdl.recompute_chunk = chunk
+            mdl.params.requires_grad_(True)
+            mdl.params.grad = None
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        pc, pf = net.forward(ro, rd, near, far, u_c, u_f)
+        (nerf.ray_utils.mse_loss(rgb, pc["pred_rgb"]) + nerf.ray_utils.mse_loss(rgb, pf["pred_rgb"])).backward()
+        torch.cuda.synchronize()
+        peaks.append(torch.cuda.max_memory_allocated())
+        grads.append([net.coarse_model.params.grad.clone(), net.fine_model.params.grad.clone()])
+    for a, b in zip(*grads):
+        assert torch.allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.abs().max()))
+    assert peaks[1] < 0.6 * peaks[0], peaks
