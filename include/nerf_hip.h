@@ -58,6 +58,20 @@ int32_t nerf_get_rays(int32_t H, int32_t W, const double* K, const double* c2w,
 int32_t nerf_get_rays_tf(int32_t H, int32_t W, const float* K, const float* c2w,
                          float* rays_o, float* rays_d, void* stream);
 
+/* Training ray batch from images resident on the GPU: the sample mode of
+ * base_dataset.py:555-621 without the host tf.data step.  images uint8
+ * (n_images, H, W, 3); poses (n_images, 4, 4), Ks (n_images, 3, 3), bounds
+ * (n_images, 2) float32 [near, far]; ray b is pixel pixel_idx[b] (int64,
+ * row-major v*W + u) of image image_idx[b] (int32), drawn by the caller
+ * (base_dataset.py:607-610 draws them with tf.random.uniform).  Outputs
+ * rays_o, rays_d (batch, 3) exactly as get_rays_tf for that pixel, near, far
+ * (batch, 1) and rgb (batch, 3) = pixel / 255 (base_dataset.py:595-598). */
+int32_t nerf_sample_batch(int64_t batch, int32_t n_images, int32_t H, int32_t W,
+                          const uint8_t* images, const float* poses, const float* Ks,
+                          const float* bounds, const int32_t* image_idx, const int64_t* pixel_idx,
+                          float* rays_o, float* rays_d, float* near, float* far, float* rgb,
+                          void* stream);
+
 /* ------------------------------------------------------------------------
  * create_input_batch_coarse_model (nerf/utils/ray_utils.py:137-274).
  *   near, far: (N,1); u_vals: (N, n_coarse) draws in [0,1) (ignored when

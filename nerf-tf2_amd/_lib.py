@@ -40,6 +40,7 @@ _SIGS = {
     "nerf_abi_version": (_i32, []),
     "nerf_get_rays": (_i32, [_i32, _i32, _dp, _dp, _vp, _vp, _vp]),
     "nerf_get_rays_tf": (_i32, [_i32, _i32, _fp, _fp, _vp, _vp, _vp]),
+    "nerf_sample_batch": (_i32, [_i64, _i32, _i32, _i32] + [_vp] * 11 + [_vp]),
     "nerf_create_input_batch_coarse": (_i32, [_i64, _i32, _i32, _i32] + [_vp] * 10 + [_vp]),
     "nerf_create_input_batch_fine": (_i32, [_i64, _i32, _i32] + [_vp] * 12 + [_vp]),
     "nerf_composite_forward": (_i32, [_i64, _i32, _i32] + [_vp] * 7 + [_vp]),

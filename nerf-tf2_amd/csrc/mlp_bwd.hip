@@ -350,6 +350,131 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
   if (do_bias && n0 + tid < a.N) a.bpart[(int64_t)split * a.N + n0 + tid] = bsum;
 }
 
+// bf16 variant for the bf16 stash: X (bf16) and dz (fp32, rounded to bf16 here)
+// are staged transposed, [feature][sample], into LDS as packed sample pairs so
+// each lane's MFMA operand (8 consecutive samples of one feature) is one
+// ds_read_b128; v_mfma_f32_32x32x16_bf16 with fp32 accumulation.  Bias sums use
+// the fp32 dz (16-lane shuffle reduction per feature group).
+constexpr int kPitch16 = kChunk + 8;  // bf16 per LDS row (16 B aligned, staggered banks)
+
+__global__ __launch_bounds__(256) void wgrad_gemm_bf16_kernel(GemmArgs a) {
+  __shared__ __attribute__((aligned(16))) __bf16 Xt[kTile][kPitch16];
+  __shared__ __attribute__((aligned(16))) __bf16 Zt[kTile][kPitch16];
+  const int tile = blockIdx.x % (a.tiles_k * a.tiles_n);
+  const int split = blockIdx.x / (a.tiles_k * a.tiles_n);
+  const int tk = tile / a.tiles_n, tn = tile % a.tiles_n;
+  const int k0 = tk * kTile, n0 = tn * kTile;
+  const int64_t mb = (int64_t)split * a.m_per_split;
+  const int64_t me = mb + a.m_per_split < a.M ? mb + a.m_per_split : a.M;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wk = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  const int r = lane & 31, q = lane >> 5;
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[x][y][i] = 0.f;
+  // staging: thread -> sample pair sp (rows 2sp, 2sp+1), feature group fg (8 features)
+  const int sp = tid & 15, fg = tid >> 4;
+  const bool do_bias = a.bpart && tk == 0;
+  float bsum[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
+  const __bf16* Xb = reinterpret_cast<const __bf16*>(a.X);
+  for (int64_t c0 = mb; c0 < me; c0 += kChunk) {
+    float xv[2][8], zv[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t m = c0 + 2 * sp + h;
+      const bool ok = m < me;
+      const int kf = k0 + 8 * fg, nf = n0 + 8 * fg;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        xv[h][e] = 0.f;
+        zv[h][e] = 0.f;
+      }
+      if (ok) {
+        const __bf16* xr = Xb + m * a.ldx + a.xoff;
+        if (kf + 7 < a.K) {
+          const bf16x8 v = *reinterpret_cast<const bf16x8*>(xr + kf);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) xv[h][e] = (float)v[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (kf + e < a.K) xv[h][e] = (float)xr[kf + e];
+        }
+        const float* zr = a.Z + m * a.ldz + a.zoff;
+        if (nf + 7 < a.N) {
+          const floatx4 v0 = *reinterpret_cast<const floatx4*>(zr + nf);
+          const floatx4 v1 = *reinterpret_cast<const floatx4*>(zr + nf + 4);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            zv[h][e] = v0[e];
+            zv[h][4 + e] = v1[e];
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (nf + e < a.N) zv[h][e] = zr[nf + e];
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+      typedef float floatx2 __attribute__((ext_vector_type(2)));
+      const floatx2 xp = {xv[0][e], xv[1][e]};
+      const floatx2 zp = {zv[0][e], zv[1][e]};
+      *reinterpret_cast<bf16x2*>(&Xt[8 * fg + e][2 * sp]) = __builtin_convertvector(xp, bf16x2);
+      *reinterpret_cast<bf16x2*>(&Zt[8 * fg + e][2 * sp]) = __builtin_convertvector(zp, bf16x2);
+      if (do_bias) bsum[e] += zv[0][e] + zv[1][e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kChunk / 16; ++s) {
+      bf16x8 av[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) av[x] = *reinterpret_cast<const bf16x8*>(&Xt[wk + 32 * x + r][16 * s + 8 * q]);
+#pragma unroll
+      for (int y = 0; y < 2; ++y) bv[y] = *reinterpret_cast<const bf16x8*>(&Zt[wn + 32 * y + r][16 * s + 8 * q]);
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = mfma_bf16(av[x], bv[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+  float* out = a.part + (int64_t)split * a.K * a.N;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int kk = k0 + wk + 32 * x + 8 * (i / 4) + 4 * q + (i % 4);
+        const int nn = n0 + wn + 32 * y + r;
+        if (kk < a.K && nn < a.N) out[(int64_t)kk * a.N + nn] = acc[x][y][i];
+      }
+  if (do_bias) {
+    // reduce over the 16 sample-pair lanes of this feature group (lanes fg*16 .. fg*16+15)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+      for (int msk = 1; msk < 16; msk <<= 1) bsum[e] += shfl_xor(bsum[e], msk);
+    }
+    if (sp == 0) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int n = n0 + 8 * fg + e;
+        if (n < a.N) a.bpart[(int64_t)split * a.N + n] = bsum[e];
+      }
+    }
+  }
+}
+
 // grad[k_off + row(k) * fan_out + n] (+)= sum_s part[s][k][n] for n < n_real;
 // row(k) = row_base + k for hidden inputs, or the encoding slot map (-1 = pad).
 struct ReduceArgs {
@@ -535,7 +660,10 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
     a.part = part;
     a.bpart = bpart;
     const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * splits;
-    wgrad_gemm_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
+    if (a.x_bf16)
+      wgrad_gemm_bf16_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
+    else
+      wgrad_gemm_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
     ReduceArgs r;
     r.part = part;
     r.bpart = bpart;

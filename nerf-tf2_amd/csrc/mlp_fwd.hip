@@ -12,7 +12,10 @@
 #include "common.h"
 #include "mlp_plan.h"
 #include "mlp_kernels.h"
+#include <type_traits>
+
 #include "mlp_fwd_bf16.h"
+#include "mlp_fwd_bf16ws.h"
 
 namespace {
 
@@ -162,6 +165,9 @@ int nerf_dispatch_fwd(const MlpPlan& P, const void* packed, int64_t M, const flo
     if (P.prec == NERF_PREC_F32)                                                              \
       return launch_fwd_f32<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr,   \
                                                   rgb, sigma, stash, s);                      \
+    if constexpr (W_ == 256 && LX_ == 10 && LD_ == 4)                                         \
+      return launch_fwd_bf16ws<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr, \
+                                                     rgb, sigma, stash, s);                   \
     return launch_fwd_bf16<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr,    \
                                                  rgb, sigma, stash, s);                       \
   }

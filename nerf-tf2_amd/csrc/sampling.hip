@@ -185,3 +185,59 @@ extern "C" int32_t nerf_create_input_batch_coarse(int64_t n_rays, int32_t n_coar
   NERF_LAUNCH_CHECK("nerf_create_input_batch_coarse");
   return NERF_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Training ray batches from images resident on the GPU (the sample mode of
+// base_dataset.py:555-621): ray b is pixel pix[b] of image img[b]; its ray is
+// get_rays_tf's (ray_utils.py:53-106) for that pixel, its colour the uint8
+// pixel / 255 (base_dataset.py:595-598), its bounds the image's (near, far).
+// ---------------------------------------------------------------------------
+__global__ void sample_batch_kernel(int64_t batch, int H, int W, const uint8_t* __restrict__ images,
+                                    const float* __restrict__ poses, const float* __restrict__ Ks,
+                                    const float* __restrict__ bounds, const int32_t* __restrict__ img,
+                                    const int64_t* __restrict__ pix, float* __restrict__ ro,
+                                    float* __restrict__ rd, float* __restrict__ near,
+                                    float* __restrict__ far, float* __restrict__ rgb) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const int i = img[b];
+  const int64_t p = pix[b];
+  const int v = (int)(p / W), u = (int)(p % W);
+  const float* K = Ks + 9 * i;
+  const float* c2w = poses + 16 * i;
+  const float x = fdiv(fsub((float)u, K[2]), K[0]);
+  const float y = fdiv(fsub((float)v, K[5]), K[4]);
+  float d[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float* R = c2w + 4 * r;
+    d[r] = fadd(fadd(fmul(x, R[0]), fmul(y, R[1])), fmul(1.0f, R[2]));
+  }
+  const float mag = fsqrt(fadd(fadd(fmul(d[0], d[0]), fmul(d[1], d[1])), fmul(d[2], d[2])));
+  const float den = fadd(mag, 1e-8f);
+  const uint8_t* px = images + (((int64_t)i * H + v) * W + u) * 3;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    rd[3 * b + r] = fdiv(d[r], den);
+    ro[3 * b + r] = c2w[4 * r + 3];
+    rgb[3 * b + r] = fdiv((float)px[r], 255.0f);
+  }
+  near[b] = bounds[2 * i];
+  far[b] = bounds[2 * i + 1];
+}
+
+extern "C" int32_t nerf_sample_batch(int64_t batch, int32_t n_images, int32_t H, int32_t W,
+                                     const uint8_t* images, const float* poses, const float* Ks,
+                                     const float* bounds, const int32_t* image_idx,
+                                     const int64_t* pixel_idx, float* rays_o, float* rays_d,
+                                     float* near, float* far, float* rgb, void* stream) {
+  NERF_CHECK_ARG(batch >= 0 && n_images > 0 && H > 0 && W > 0, "nerf_sample_batch: bad sizes");
+  NERF_CHECK_ARG(images && poses && Ks && bounds && image_idx && pixel_idx && rays_o && rays_d && near &&
+                     far && rgb,
+                 "nerf_sample_batch: null pointer");
+  if (batch == 0) return NERF_OK;
+  sample_batch_kernel<<<(unsigned)((batch + 255) / 256), 256, 0, as_stream(stream)>>>(
+      batch, H, W, images, poses, Ks, bounds, image_idx, pixel_idx, rays_o, rays_d, near, far, rgb);
+  NERF_LAUNCH_CHECK("nerf_sample_batch");
+  return NERF_OK;
+}

@@ -42,7 +42,9 @@ def make_batches(nerf, scene, n_steps, batch=4096, n_views=100, H=800, W=800, fo
     ro_all = torch.cat(ro_all)              # (V, 3)
     g = torch.Generator(device=device).manual_seed(seed)
     out = []
-    for _ in range(n_steps):
+    for step in range(n_steps):
+        if step % 500 == 0:
+            print(f"quality: ground truth for batch {step}/{n_steps}", file=sys.stderr, flush=True)
         v = torch.randint(0, n_views, (batch,), device=device, generator=g)
         p = torch.randint(0, H * W, (batch,), device=device, generator=g)
         o = ro_all[v].contiguous()
@@ -88,7 +90,9 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda"):
         t0 = time.perf_counter()
         for i in range(steps):
             o, d, gt = batches[i]
-            net.train_step(o, d, near, far, gt)
+            st = net.train_step(o, d, near, far, gt)
+            if i % 250 == 0 or i == steps - 1:
+                print(f"quality: {name} step {i} fine_loss {float(st['fine_loss']):.5f}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         preds, fb = [], []
@@ -103,6 +107,7 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda"):
         torch.cuda.synchronize()
         t_render = time.perf_counter() - t0
         renders[name] = preds
+        print(f"quality: {name} rendered {len(preds)} views", file=sys.stderr, flush=True)
         gts = [v[2] for v in views]
         res[name] = {
             "train_rays_per_s": round(4096 * steps / t_train, 1),

@@ -241,3 +241,58 @@ def interlevel(prop_edges: Tensor, prop_weights: Tensor, t_fine: Tensor, fine_we
          ptr(_f32(t_fine)), ptr(_f32(fine_weights.detach())), ptr(_f32(far).reshape(n, 1)), ptr(missed),
          ptr(loss), ptr(g), float(grad_scale), stream(dev))
     return missed, loss, g
+
+
+class RayBatchSampler:
+    """The reference's sample mode (base_dataset.py:555-621) on the GPU: images
+    stay resident as uint8 (N, H, W, 3) with their poses, intrinsics and bounds;
+    each call draws `batch` (image, pixel) pairs on the device and builds the
+    rays exactly as get_rays_tf would for those pixels, with rgb = pixel / 255.
+
+    ``one_image_per_batch=True`` mirrors the reference (one image per step,
+    base_dataset.py:607-610); False draws every ray's image independently."""
+
+    def __init__(self, images: Tensor, poses, intrinsics, bounds, batch: int = 4096,
+                 one_image_per_batch: bool = True, seed: int = 0):
+        if images.dtype != torch.uint8 or images.dim() != 4 or images.shape[-1] != 3:
+            raise ValueError("images must be uint8 (N, H, W, 3)")
+        dev = images.device
+        self.images = images.contiguous()
+        n, self.H, self.W, _ = images.shape
+        self.n = n
+        self.poses = torch.as_tensor(np.asarray(poses, dtype=np.float32).reshape(n, 4, 4)).to(dev).contiguous()
+        K = np.asarray(intrinsics, dtype=np.float32)
+        if K.shape == (3, 3):
+            K = np.broadcast_to(K, (n, 3, 3))
+        self.Ks = torch.as_tensor(np.ascontiguousarray(K)).to(dev).contiguous()
+        B = np.asarray(bounds, dtype=np.float32)
+        if B.shape == (2,):
+            B = np.broadcast_to(B, (n, 2))
+        self.bounds = torch.as_tensor(np.ascontiguousarray(B)).to(dev).contiguous()
+        self.batch = batch
+        self.one_image = one_image_per_batch
+        self.gen = torch.Generator(device=dev).manual_seed(seed)
+
+    def sample(self, image_idx: Optional[Tensor] = None, pixel_idx: Optional[Tensor] = None):
+        """Returns ((rays_o, rays_d, near, far), (rgb,)) like the reference's dataset element."""
+        dev = self.images.device
+        b = self.batch
+        if image_idx is None:
+            if self.one_image:
+                image_idx = torch.randint(0, self.n, (1,), device=dev, generator=self.gen,
+                                          dtype=torch.int32).expand(b).contiguous()
+            else:
+                image_idx = torch.randint(0, self.n, (b,), device=dev, generator=self.gen, dtype=torch.int32)
+        if pixel_idx is None:
+            pixel_idx = torch.randint(0, self.H * self.W, (b,), device=dev, generator=self.gen, dtype=torch.int64)
+        image_idx = image_idx.to(torch.int32).contiguous()
+        pixel_idx = pixel_idx.to(torch.int64).contiguous()
+        ro = torch.empty((b, 3), dtype=torch.float32, device=dev)
+        rd = torch.empty_like(ro)
+        rgb = torch.empty_like(ro)
+        near = torch.empty((b, 1), dtype=torch.float32, device=dev)
+        far = torch.empty_like(near)
+        call("nerf_sample_batch", b, self.n, self.H, self.W, ptr(self.images), ptr(self.poses), ptr(self.Ks),
+             ptr(self.bounds), ptr(image_idx), ptr(pixel_idx), ptr(ro), ptr(rd), ptr(near), ptr(far), ptr(rgb),
+             stream(dev))
+        return (ro, rd, near, far), (rgb,)

@@ -464,3 +464,20 @@ def psnr(y_true: Tensor, y_pred: Tensor) -> float:
     """Per-element PSNR, as ops.py:249-257 (psnr_metric_numpy)."""
     mse = float(torch.mean((y_true.double() - y_pred.double()) ** 2))
     return -10.0 * math.log10(mse)
+
+
+# --------------------------------------------------------------------------
+# Sample-mode ray batches (nerf/core/base_dataset.py:555-621)
+# --------------------------------------------------------------------------
+def sample_mode_map(img: np.ndarray, pose: Tensor, bounds: Tensor, intrinsic: Tensor, idxs: Tensor):
+    """base_dataset.py:555-621 for one image with the pixel draws injected
+    (``idxs`` replaces the tf.random.uniform of :607-610).  Returns
+    ((rays_o, rays_d, near, far), (rgb,))."""
+    H, W = img.shape[0], img.shape[1]
+    ro, rd = get_rays_tf(H, W, intrinsic, pose)
+    rgb = torch.from_numpy(np.ascontiguousarray(img)).reshape(-1, 3).to(torch.float32)
+    rgb = rgb / torch.tensor(255.0, dtype=torch.float32)
+    b = bounds.reshape(1, 2).expand(ro.shape[0], 2)
+    near, far = b[:, 0:1], b[:, 1:2]
+    sel = idxs.long()
+    return (ro[sel], rd[sel], near[sel], far[sel]), (rgb[sel],)

@@ -276,3 +276,33 @@ def test_nerf_forward_config0(nerf, oracle):
                                                       dC["t_vals"].to(DEV), u_vals=u_f, return_indices=True)
     assert torch.equal(df["piece_idxs"].cpu(), dF["piece_idxs"])
     assert torch.equal(df["t_vals"].cpu(), dF["t_vals"])
+
+
+def test_sample_batch_producer(nerf, oracle):
+    """GPU sample mode (base_dataset.py:555-621) against the oracle, bit-exact."""
+    sc = nerf.scene
+    n_img, H, W = 5, 60, 80
+    rng = np.random.default_rng(3)
+    images = rng.integers(0, 256, size=(n_img, H, W, 3), dtype=np.uint8)
+    poses = sc.random_poses(n_img, 11).astype(np.float32)
+    K = sc.intrinsics(H, W, 70.0).astype(np.float32)
+    bounds = np.array([2.0, 6.0], dtype=np.float32)
+    smp = nerf.ray_utils.RayBatchSampler(torch.from_numpy(images).to(DEV), poses, K, bounds, batch=777,
+                                         one_image_per_batch=False)
+    img_idx = torch.from_numpy(rng.integers(0, n_img, size=777).astype(np.int32)).to(DEV)
+    pix_idx = torch.from_numpy(rng.integers(0, H * W, size=777)).to(DEV)
+    (ro, rd, near, far), (rgb,) = smp.sample(img_idx, pix_idx)
+    for i in range(n_img):
+        sel = (img_idx == i).cpu()
+        if not bool(sel.any()):
+            continue
+        (o, d, nr, fr), (c,) = oracle.sample_mode_map(images[i], torch.from_numpy(poses[i]),
+                                                     torch.from_numpy(bounds), torch.from_numpy(K),
+                                                     pix_idx.cpu()[sel])
+        assert torch.equal(ro.cpu()[sel], o) and torch.equal(rd.cpu()[sel], d)
+        assert torch.equal(near.cpu()[sel], nr) and torch.equal(far.cpu()[sel], fr)
+        assert torch.equal(rgb.cpu()[sel], c)
+    # one image per batch, drawn on the device
+    smp1 = nerf.ray_utils.RayBatchSampler(torch.from_numpy(images).to(DEV), poses, K, bounds, batch=256)
+    (ro1, _, _, _), _ = smp1.sample()
+    assert bool((ro1 == ro1[0:1]).all())

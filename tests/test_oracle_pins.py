@@ -436,3 +436,21 @@ def sc_rays(sc, pose, H=8, W=8, f=10.0):
     o = importlib.import_module("oracle.restatement")
     ro, rd = o.get_rays(H, W, K, pose)
     return torch.from_numpy(np.ascontiguousarray(ro)), torch.from_numpy(rd)
+
+
+def test_sample_mode_map(oracle):
+    """base_dataset.py:555-621 restated: rays are get_rays_tf's rows, rgb = u8/255."""
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 256, size=(6, 7, 3), dtype=np.uint8)
+    K = torch.tensor([[5.0, 0, 3.5], [0, 5.0, 3.0], [0, 0, 1]], dtype=torch.float32)
+    pose = torch.eye(4, dtype=torch.float32)
+    pose[:3, 3] = torch.tensor([0.5, -1.0, -4.0])
+    idx = torch.tensor([0, 41, 7, 7, 20])
+    (o, d, n, f), (c,) = oracle.sample_mode_map(img, pose, torch.tensor([2.0, 6.0]), K, idx)
+    ro, rd = oracle.get_rays_tf(6, 7, K, pose)
+    assert torch.equal(d, rd[idx]) and torch.equal(o, ro[idx])
+    assert torch.equal(c, torch.from_numpy(img.reshape(-1, 3)[idx.numpy()].astype(np.float32)) / 255.0)
+    assert torch.equal(n, torch.full((5, 1), 2.0)) and torch.equal(f, torch.full((5, 1), 6.0))
+    # pixel 41 = (v, u) = (5, 6): direction before normalisation ((6-3.5)/5, (5-3)/5, 1)
+    v = np.array([(6 - 3.5) / 5, (5 - 3) / 5, 1.0])
+    assert np.allclose(d[1].numpy(), v / np.linalg.norm(v), atol=1e-6)
