@@ -244,6 +244,8 @@ int32_t nerf_f64_dense_wgrad(int64_t M, int32_t K, const double* X, int32_t N, c
  * proposal weights on its bins (prop_edges (N, n_prop+1), prop_weights
  * (N, n_prop)) and the fine weights fine_weights (N, n_fine) at the sorted
  * sample positions t_fine (N, n_fine); the last fine interval ends at far (N,1).
+ * The bound of a fine interval sums the proposal bins overlapping it, widened
+ * by `dilate` bins on each side (0: the plain outer measure).
  * Outputs (each may be NULL): missed (N,) missed weight mass per ray, loss (N,)
  * per-ray loss, g_prop_weights (N, n_prop) = grad_scale * d loss / d prop_weights.
  * ---------------------------------------------------------------------- */
@@ -251,7 +253,7 @@ int32_t nerf_proposal_interlevel(int64_t n_rays, int32_t n_prop, int32_t n_fine,
                                  const float* prop_edges, const float* prop_weights,
                                  const float* t_fine, const float* fine_weights, const float* far,
                                  float* missed, float* loss, float* g_prop_weights,
-                                 float grad_scale, void* stream);
+                                 float grad_scale, int32_t dilate, void* stream);
 
 #ifdef __cplusplus
 }

@@ -252,8 +252,13 @@ struct GemmArgs {
 __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
   __shared__ float Xs[kChunk][kTile + kPad];
   __shared__ float Zs[kChunk][kTile + kPad];
-  const int tile = blockIdx.x % (a.tiles_k * a.tiles_n);
-  const int split = blockIdx.x / (a.tiles_k * a.tiles_n);
+  // XCD-aware order: workgroup b runs on XCD b % 8, so all tiles of one sample
+  // split (which read the same X / dz rows) share b % 8 and hence one L2.
+  const int T = a.tiles_k * a.tiles_n;
+  const int xcd = blockIdx.x % 8, yy = blockIdx.x / 8;
+  const int tile = yy % T;
+  const int split = (yy / T) * 8 + xcd;
+  if (split >= a.splits) return;  // padding of the split count to a multiple of 8
   const int tk = tile / a.tiles_n, tn = tile % a.tiles_n;
   const int k0 = tk * kTile, n0 = tn * kTile;
   const int64_t mb = (int64_t)split * a.m_per_split;
@@ -360,8 +365,13 @@ constexpr int kPitch16 = kChunk + 8;  // bf16 per LDS row (16 B aligned, stagger
 __global__ __launch_bounds__(256) void wgrad_gemm_bf16_kernel(GemmArgs a) {
   __shared__ __attribute__((aligned(16))) __bf16 Xt[kTile][kPitch16];
   __shared__ __attribute__((aligned(16))) __bf16 Zt[kTile][kPitch16];
-  const int tile = blockIdx.x % (a.tiles_k * a.tiles_n);
-  const int split = blockIdx.x / (a.tiles_k * a.tiles_n);
+  // XCD-aware order: workgroup b runs on XCD b % 8, so all tiles of one sample
+  // split (which read the same X / dz rows) share b % 8 and hence one L2.
+  const int T = a.tiles_k * a.tiles_n;
+  const int xcd = blockIdx.x % 8, yy = blockIdx.x / 8;
+  const int tile = yy % T;
+  const int split = (yy / T) * 8 + xcd;
+  if (split >= a.splits) return;  // padding of the split count to a multiple of 8
   const int tk = tile / a.tiles_n, tn = tile % a.tiles_n;
   const int k0 = tk * kTile, n0 = tn * kTile;
   const int64_t mb = (int64_t)split * a.m_per_split;
@@ -659,7 +669,7 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
     a.splits = (int)splits;
     a.part = part;
     a.bpart = bpart;
-    const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * splits;
+    const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * ((splits + 7) / 8) * 8;
     if (a.x_bf16)
       wgrad_gemm_bf16_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
     else

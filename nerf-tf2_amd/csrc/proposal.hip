@@ -5,8 +5,9 @@
 //
 // For fine interval i = [t_i, t_{i+1}] (the last one ends at `far`), the
 // proposal's outer measure is bound_i = sum of w_p over the proposal bins that
-// overlap it, summed directly over the bin range (a difference of two float32
-// cdf values would cancel for the many tiny fine weights).  missed_i = max(0, w_i - bound_i); the ray's missed mass is
+// overlap it, widened by `dilate` bins on each side (a blurred histogram; 0 is
+// the plain outer measure), summed directly over the bin range (a difference
+// of two float32 cdf values would cancel for the many tiny fine weights).  missed_i = max(0, w_i - bound_i); the ray's missed mass is
 // sum_i missed_i and its loss sum_i missed_i^2 / (w_i + 1e-7), with w_i held
 // constant.  d loss / d w_p[k] = sum over intervals i whose bin range holds k of
 // -2 missed_i / (w_i + 1e-7), summed by one lane per bin in a fixed order.
@@ -20,7 +21,8 @@ constexpr int kMaxSp = 64, kMaxSf = 256;
 __global__ __launch_bounds__(256) void interlevel_kernel(
     int64_t n_rays, int Sp, int Sf, const float* __restrict__ edges, const float* __restrict__ wp,
     const float* __restrict__ tf, const float* __restrict__ wf, const float* __restrict__ far,
-    float* __restrict__ missed, float* __restrict__ loss, float* __restrict__ g_wp, float gscale) {
+    float* __restrict__ missed, float* __restrict__ loss, float* __restrict__ g_wp, float gscale,
+    int dilate) {
   __shared__ float s_e[kW][kMaxSp + 1], s_w[kW][kMaxSp], s_coef[kW][kMaxSf];
   __shared__ int s_lo[kW][kMaxSf], s_hi[kW][kMaxSf];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -52,6 +54,8 @@ __global__ __launch_bounds__(256) void interlevel_kernel(
     }
     int khi = a;
     khi = khi > Sp ? Sp : (khi < klo + 1 ? klo + 1 : khi);
+    klo = klo - dilate < 0 ? 0 : klo - dilate;
+    khi = khi + dilate > Sp ? Sp : khi + dilate;
     float bound = 0.f;
     for (int k = klo; k < khi; ++k) bound += s_w[w][k];
     const float wi = wf[ray * Sf + i];
@@ -86,16 +90,18 @@ extern "C" int32_t nerf_proposal_interlevel(int64_t n_rays, int32_t n_prop, int3
                                             const float* prop_edges, const float* prop_weights,
                                             const float* t_fine, const float* fine_weights,
                                             const float* far, float* missed, float* loss,
-                                            float* g_prop_weights, float grad_scale, void* stream) {
+                                            float* g_prop_weights, float grad_scale, int32_t dilate,
+                                            void* stream) {
   NERF_CHECK_ARG(n_rays >= 0 && n_prop >= 1 && n_prop <= kMaxSp && n_fine >= 1 && n_fine <= kMaxSf,
                  "nerf_proposal_interlevel: n_prop=%d (1..%d) n_fine=%d (1..%d)", n_prop, kMaxSp,
                  n_fine, kMaxSf);
   NERF_CHECK_ARG(prop_edges && prop_weights && t_fine && fine_weights && far,
                  "nerf_proposal_interlevel: null pointer");
+  NERF_CHECK_ARG(dilate >= 0 && dilate < n_prop, "nerf_proposal_interlevel: dilate=%d", dilate);
   if (n_rays == 0) return NERF_OK;
   interlevel_kernel<<<(unsigned)((n_rays + kW - 1) / kW), 64 * kW, 0, as_stream(stream)>>>(
       n_rays, n_prop, n_fine, prop_edges, prop_weights, t_fine, fine_weights, far, missed, loss,
-      g_prop_weights, grad_scale);
+      g_prop_weights, grad_scale, dilate);
   NERF_LAUNCH_CHECK("nerf_proposal_interlevel");
   return NERF_OK;
 }

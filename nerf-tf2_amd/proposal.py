@@ -10,8 +10,10 @@ Per ray:
   3. 128 fine samples from the proposal weights (the same CDF resampler);
   4. the fine 8x256 model on the 48 + 128 sorted samples -> rgb / depth / acc;
   5. the proposal's missed weight mass against the fine weights (interlevel
-     kernel).  Rays missing more than `fallback_threshold` are re-rendered by
-     the plain coarse + fine pass (64 + 128 samples) on a compacted subset.
+     kernel; the proposal's histogram is blurred by `dilate` bins on each side,
+     in the loss and in this test alike).  Rays missing more than
+     `fallback_threshold` are re-rendered by the plain coarse + fine pass
+     (64 + 128 samples) on a compacted subset.
 Training: the fine model on its photometric loss; the proposal online on the
 histogram upper-bound loss against the detached fine weights; the coarse
 model (the fallback) on its photometric loss over a strided subset of the
@@ -46,7 +48,8 @@ class _P:
 
 class ProposalNeRF:
     def __init__(self, params, precision: str = "bf16", seed: int = 0, device="cuda", n_prop: int = 48,
-                 fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 1.0):
+                 fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 1.0,
+                 dilate: int = 1):
         self.params = params
         self.white_bg = bool(params.system.white_bg)
         self.precision = precision
@@ -56,6 +59,7 @@ class ProposalNeRF:
         self.fallback_threshold = fallback_threshold
         self.coarse_stride = coarse_stride
         self.interlevel_weight = interlevel_weight
+        self.dilate = dilate
         self.p_prop = _P(n_prop, self.n_fine, params.sampling.lin_inv_depth, self.white_bg)
         self.proposal = NerfMLP("proposal", PROPOSAL_CONFIG, precision, seed=seed + 2, device=device)
         self.coarse_model = NerfMLP("coarse", MLPConfig(), precision, seed=seed, device=device)
@@ -86,7 +90,7 @@ class ProposalNeRF:
         the plain coarse + fine pass (their weights stay those of the proposal pass)."""
         dp, pp_p, df, pp_f = self._proposal_fine(rays_o, rays_d, near, far, u_prop, u_fine)
         missed, _, _ = ray_utils.interlevel(dp["bin_data"]["bin_edges"], pp_p["weights"], df["t_vals"],
-                                            pp_f["weights"], far)
+                                            pp_f["weights"], far, dilate=self.dilate)
         pp_f["t_vals"] = df["t_vals"]
         if fallback:
             idx = torch.nonzero(missed > self.fallback_threshold).flatten()
@@ -124,7 +128,7 @@ class ProposalNeRF:
         # proposal: d(mean interlevel loss)/d w_p from the kernel, applied through a surrogate
         missed, il, g_wp = ray_utils.interlevel(dp["bin_data"]["bin_edges"], pp_p["weights"], df["t_vals"],
                                                 pp_f["weights"], far, want_grad=True,
-                                                grad_scale=self.interlevel_weight / n)
+                                                grad_scale=self.interlevel_weight / n, dilate=self.dilate)
         surrogate = (pp_p["weights"] * g_wp).sum()
         # coarse (fallback) model on a strided subset of the batch
         sub = slice(0, n, self.coarse_stride)

@@ -59,7 +59,7 @@ def psnr(a, b):
     return -10.0 * math.log10(max(mse, 1e-30))
 
 
-def run(steps=2000, eval_views=2, seed=0, device="cuda"):
+def run(steps=2000, eval_views=2, seed=0, device="cuda", forms_sel=("plain_fp32", "plain_bf16", "learned_bf16")):
     nerf = _pkg()
     sc = nerf.scene
     scene = sc.BlobScene.from_seed(0)
@@ -82,6 +82,7 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda"):
         "plain_bf16": lambda: nerf.NeRF(P, "bf16", seed=seed, device=device),
         "learned_bf16": lambda: nerf.ProposalNeRF(P, "bf16", seed=seed, device=device),
     }
+    forms = {k: v for k, v in forms.items() if k in forms_sel}
     res = {"steps": steps, "batch": 4096, "data_seconds": round(t_data, 1)}
     renders = {}
     for name, make in forms.items():
@@ -123,6 +124,8 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda"):
         del net
         torch.cuda.empty_cache()
     for name in ("plain_bf16", "learned_bf16"):
+        if name not in res or "plain_fp32" not in res:
+            continue
         res[name]["psnr_vs_plain_fp32"] = [round(psnr(a, b), 3) for a, b in zip(renders[name], renders["plain_fp32"])]
         res[name]["max_abs_rgb_err_vs_plain_fp32"] = [round(float((a - b).abs().max()), 4)
                                                       for a, b in zip(renders[name], renders["plain_fp32"])]
@@ -139,8 +142,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--eval-views", type=int, default=2)
+    ap.add_argument("--forms", default="plain_fp32,plain_bf16,learned_bf16")
     args = ap.parse_args(argv)
-    print(json.dumps(run(args.steps, args.eval_views)))
+    print(json.dumps(run(args.steps, args.eval_views, forms_sel=tuple(args.forms.split(",")))))
 
 
 if __name__ == "__main__":

@@ -228,8 +228,9 @@ def mse_loss(y_true: Tensor, y_pred: Tensor) -> Tensor:
 
 
 def interlevel(prop_edges: Tensor, prop_weights: Tensor, t_fine: Tensor, fine_weights: Tensor, far: Tensor,
-               want_grad: bool = False, grad_scale: float = 1.0):
-    """Histogram upper-bound loss of the proposal against the fine weights.
+               want_grad: bool = False, grad_scale: float = 1.0, dilate: int = 0):
+    """Histogram upper-bound loss of the proposal against the fine weights
+    (bound over the overlapping proposal bins widened by `dilate` on each side).
     Returns (missed (N,), loss (N,), d loss / d prop_weights (N, n_prop) * grad_scale or None)."""
     n, Sp = prop_weights.shape
     Sf = t_fine.shape[1]
@@ -239,7 +240,7 @@ def interlevel(prop_edges: Tensor, prop_weights: Tensor, t_fine: Tensor, fine_we
     g = torch.empty((n, Sp), dtype=torch.float32, device=dev) if want_grad else None
     call("nerf_proposal_interlevel", n, Sp, Sf, ptr(_f32(prop_edges)), ptr(_f32(prop_weights.detach())),
          ptr(_f32(t_fine)), ptr(_f32(fine_weights.detach())), ptr(_f32(far).reshape(n, 1)), ptr(missed),
-         ptr(loss), ptr(g), float(grad_scale), stream(dev))
+         ptr(loss), ptr(g), float(grad_scale), int(dilate), stream(dev))
     return missed, loss, g
 
 

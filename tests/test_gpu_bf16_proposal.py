@@ -94,7 +94,7 @@ def test_bf16_mlp_forward_and_backward(nerf, oracle):
     assert float((rgb.detach().cpu() - rgb_r).abs().max()) < 3e-2
 
 
-def _interlevel_reference(edges, wp, tf, wf, far):
+def _interlevel_reference(edges, wp, tf, wf, far, dilate=0):
     """Float64 restatement: bound_i = sum of w_p over the bins overlapping
     [t_i, t_{i+1}] (last: [t_last, far]); missed = sum max(0, w - bound);
     loss = sum max(0, w - bound)^2 / (w + 1e-7)."""
@@ -112,6 +112,7 @@ def _interlevel_reference(edges, wp, tf, wf, far):
             klo = min(max(klo, 0), Sp - 1)
             khi = int((e < hi).sum())
             khi = min(max(khi, klo + 1), Sp)
+            klo, khi = max(klo - dilate, 0), min(khi + dilate, Sp)
             b = c[khi] - c[klo]
             m = torch.clamp(wf[r, i] - b, min=0.0)
             missed[r] = missed[r] + m
@@ -119,7 +120,8 @@ def _interlevel_reference(edges, wp, tf, wf, far):
     return missed, loss
 
 
-def test_interlevel_kernel(nerf):
+@pytest.mark.parametrize("dilate", [0, 1])
+def test_interlevel_kernel(nerf, dilate):
     n, Sp, Sf = 24, 48, 176
     g = torch.Generator().manual_seed(3)
     edges = torch.sort(2 + 4 * torch.rand(n, Sp + 1, generator=g), 1).values
@@ -131,9 +133,9 @@ def test_interlevel_kernel(nerf):
     wf = wf / wf.sum(1, keepdim=True) * 0.95
     far = torch.full((n, 1), 6.0)
     missed, loss, gw = nerf.ray_utils.interlevel(edges.to(DEV), wp.to(DEV), tf.to(DEV), wf.to(DEV), far.to(DEV),
-                                                want_grad=True, grad_scale=1.0)
+                                                want_grad=True, grad_scale=1.0, dilate=dilate)
     wp64 = wp.double().requires_grad_(True)
-    m_ref, l_ref = _interlevel_reference(edges.double(), wp64, tf.double(), wf.double(), far.double())
+    m_ref, l_ref = _interlevel_reference(edges.double(), wp64, tf.double(), wf.double(), far.double(), dilate)
     g_ref, = torch.autograd.grad(l_ref.sum(), wp64)
     assert torch.allclose(missed.cpu().double(), m_ref.detach(), rtol=1e-4, atol=1e-6)
     assert torch.allclose(loss.cpu().double(), l_ref.detach(), rtol=1e-3, atol=1e-6)
