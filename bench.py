@@ -37,12 +37,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--train-steps", type=int, default=10)
+    ap.add_argument("--train-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--forms", default="plain_fp32,plain_bf16,learned_bf16",
                     help="comma list; the first is the headline `value`")
-    ap.add_argument("--learned-warmup-steps", type=int, default=300,
+    ap.add_argument("--learned-warmup-steps", type=int, default=2000,
                     help="train steps of the learned form before its render is timed")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
@@ -87,7 +87,7 @@ def main():
     poses = sc.random_poses(100, 7)
     rng = np.random.default_rng(0)
     batches = []
-    for b in range(4):
+    for b in range(16):
         views = rng.integers(0, 100, size=4096)
         pix = rng.integers(0, H * W, size=4096)
         o_l, d_l = [], []
@@ -145,20 +145,35 @@ def main():
 
 
 def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
-    """Render (timed, K steps of one 800x800 view) and train (timed) rays/s of one form."""
+    """Train (timed, graph-replayed 4096-ray steps) and render (timed, K steps of
+    one 800x800 view) rays/s of one form.  The learned form trains its warm-up
+    steps first so that its render runs with a trained proposal."""
     import numpy as np
     import torch
     n_rays = ro.shape[0]
     learned = form.startswith("learned")
     prec = "bf16" if form.endswith("bf16") else "fp32"
-    if learned:
-        net = nerf.ProposalNeRF(P, prec, seed=0, device=dev)
-        for i in range(args.learned_warmup_steps):   # let the proposal learn before timing its render
-            net.train_step(*batches[i % 4])
+    net = nerf.ProposalNeRF(P, prec, seed=0, device=dev) if learned else nerf.NeRF(P, prec, seed=0, device=dev)
+    out = {"train_rays_per_s": None, "train_ms_per_step": None}
+    nb = len(batches)
+    if args.train_steps > 0 or learned:
+        step = nerf.GraphedTrainStep(net, batches[0][0].shape[0])
+        pre = args.learned_warmup_steps if learned else 2
+        for i in range(pre):
+            step(*batches[i % nb])
+            if (i + 1) % 500 == 0:
+                print(f"bench: {form} warm-up train step {i + 1}/{pre}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
-    else:
-        net = nerf.NeRF(P, prec, seed=0, device=dev)
-    mlp_ms = []
+        if args.train_steps > 0:
+            t0 = time.perf_counter()
+            for i in range(args.train_steps):
+                step(*batches[(pre + i) % nb])
+            torch.cuda.synchronize()
+            t_train = time.perf_counter() - t0
+            out["train_rays_per_s"] = round(batches[0][0].shape[0] * args.train_steps / t_train, 1)
+            out["train_ms_per_step"] = round(1e3 * t_train / args.train_steps, 3)
+        del step
+    events = []
 
     def render(timed_kernel):
         fb = 0.0
@@ -169,37 +184,25 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
                     net.forward(ro[sl], rd[sl], near[sl], far[sl], fallback=True)
                 fb += net.last_fallback_fraction * (sl.stop - sl.start)
             else:
-                _render_chunk(net, nerf, ro[sl], rd[sl], near[sl], far[sl], timed_kernel, mlp_ms)
+                _render_chunk(net, nerf, ro[sl], rd[sl], near[sl], far[sl], timed_kernel, events)
         return fb / n_rays
 
     for _ in range(args.warmup):
         render(False)
     torch.cuda.synchronize()
-    mlp_ms.clear()
+    events.clear()
     t0 = time.perf_counter()
     fb = 0.0
     for _ in range(args.steps):
         fb = render(True)
     torch.cuda.synchronize()
     t_render = time.perf_counter() - t0
-    out = {"render_rays_per_s": round(n_rays * args.steps / t_render, 1),
-           "render_ms_per_step": round(1e3 * t_render / args.steps, 3),
-           "train_rays_per_s": None, "train_ms_per_step": None}
-    if args.train_steps > 0:
-        for i in range(2):
-            net.train_step(*batches[i % 4])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(args.train_steps):
-            net.train_step(*batches[i % 4])
-        torch.cuda.synchronize()
-        t_train = time.perf_counter() - t0
-        out["train_rays_per_s"] = round(4096 * args.train_steps / t_train, 1)
-        out["train_ms_per_step"] = round(1e3 * t_train / args.train_steps, 3)
+    out["render_rays_per_s"] = round(n_rays * args.steps / t_render, 1)
+    out["render_ms_per_step"] = round(1e3 * t_render / args.steps, 3)
     if learned:
         out["fallback_fraction"] = round(fb, 4)
         out["warmup_train_steps"] = args.learned_warmup_steps
-    full = [ms for ms, m in mlp_ms if m == chunk * 192]
+    full = [e0.elapsed_time(e1) for e0, e1, m in events if m == chunk * 192]
     if full:
         avg_ms = float(np.mean(full))
         flop = chunk * 192 * MLP_FLOP_PER_SAMPLE
@@ -211,8 +214,9 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
     return out
 
 
-def _render_chunk(net, nerf, ro, rd, near, far, timed_kernel, mlp_ms):
-    """NeRF.forward for one chunk with the fine MLP launch bracketed by events."""
+def _render_chunk(net, nerf, ro, rd, near, far, timed_kernel, events):
+    """NeRF.forward for one chunk with the fine MLP launch bracketed by HIP events
+    on the stream it is launched on (read after the timed region)."""
     import torch
     ru = nerf.ray_utils
     with torch.no_grad():
@@ -227,10 +231,8 @@ def _render_chunk(net, nerf, ro, rd, near, far, timed_kernel, mlp_ms):
         rgb_f, sig_f = net.fine_model.forward_rays(ro, rd, df["t_vals"])
         if timed_kernel:
             e1.record()
+            events.append((e0, e1, int(df["t_vals"].numel())))
         pf = ru.post_process_model_output(rgb_f, sig_f, df["t_vals"], net.white_bg)
-        if timed_kernel:
-            e1.synchronize()
-            mlp_ms.append((e0.elapsed_time(e1), int(df["t_vals"].numel())))
     return pf
 
 

@@ -217,6 +217,44 @@ int32_t nerf_adam_tf(int64_t n, float* var, float* m, float* v, const float* gra
 int32_t nerf_check_finite(int64_t n, const float* x, int32_t* flag, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Device-resident state of a loss-scaled training step, so that a BF16 train
+ * step never waits on the host: TF's LossScaleOptimizer (dynamic scale,
+ * skip on overflow, the skipped step does not advance `iterations`) around
+ * Keras Adam + ExponentialDecay (nerf/core/model.py:413-418).  The reference
+ * trains in FP32 and has no loss scaler; this is the state of the BF16 form.
+ * 64 bytes, caller-allocated in device memory, zero-filled, then `scale` set.
+ *   nerf_step_begin: overflow = 0, inv_scale = 1/scale, and the Adam
+ *     coefficients of step `iterations` (lr = lr0 * rate^(it/decay_steps),
+ *     lr_t = lr * sqrt(1 - b2^(it+1)) / (1 - b1^(it+1)), float32 operands,
+ *     powers evaluated in float64 and rounded once).
+ *   nerf_check_finite(n, grad, &state->overflow, ...) between the two.
+ *   nerf_adam_tf_state: nerf_adam_tf with lr_t, 1-b1, 1-b2, inv_scale and the
+ *     skip flag taken from the state.
+ *   nerf_step_end: on overflow scale = max(scale/factor, min_scale), good = 0,
+ *     skipped += 1; else iterations += 1, good += 1 and, after `interval`
+ *     clean steps, scale *= factor, good = 0.
+ * ---------------------------------------------------------------------- */
+typedef struct nerf_step_state {
+  float scale;
+  float inv_scale;
+  int32_t overflow;
+  int32_t good;
+  int32_t iterations;
+  int32_t skipped;
+  float lr_t;
+  float one_minus_b1;
+  float one_minus_b2;
+  int32_t reserved[7];
+} nerf_step_state;
+
+int32_t nerf_step_begin(nerf_step_state* state, float lr0, int32_t decay_steps, float decay_rate,
+                        float b1, float b2, void* stream);
+int32_t nerf_adam_tf_state(int64_t n, float* var, float* m, float* v, const float* grad, float eps,
+                           const nerf_step_state* state, void* stream);
+int32_t nerf_step_end(nerf_step_state* state, int32_t interval, float factor, float min_scale,
+                      void* stream);
+
+/* ------------------------------------------------------------------------
  * Float64 layer kernels for the finite-difference check of the hand-written
  * backward (the FP64 gradient check; no counterpart in the reference, whose
  * gradients come from TF's GradientTape over model.py:355-392).  Row-major

@@ -60,6 +60,9 @@ _SIGS = {
     "nerf_mse_grad": (_i32, [_i64, _vp, _vp, _f32, _vp, _vp, _vp]),
     "nerf_adam_tf": (_i32, [_i64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _vp, _vp, _vp]),
     "nerf_check_finite": (_i32, [_i64, _vp, _vp, _vp]),
+    "nerf_step_begin": (_i32, [_vp, _f32, _i32, _f32, _f32, _f32, _vp]),
+    "nerf_adam_tf_state": (_i32, [_i64, _vp, _vp, _vp, _vp, _f32, _vp, _vp]),
+    "nerf_step_end": (_i32, [_vp, _i32, _f32, _f32, _vp]),
     "nerf_proposal_interlevel": (_i32, [_i64, _i32, _i32] + [_vp] * 8 + [_f32, _i32, _vp]),
     "nerf_f64_encode": (_i32, [_i64, _i32, _vp, _vp, _vp]),
     "nerf_f64_dense_forward": (_i32, [_i64, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp]),

@@ -1,16 +1,30 @@
 // Weight-stationary BF16 fused forward for the reference topology (W = 256).
 //
 // A workgroup of 4 waves carries 128 samples (4 blocks of 32) through the MLP.
-// For each layer, wave w holds output blocks {2w, 2w+1} of the layer's packed
-// bf16 weights in VGPRs (16 k-steps x 2 blocks x 16 B per lane) and streams
-// the B operands of all 4 sample blocks from an LDS activation buffer, so each
-// weight byte fetched from L2 feeds 128 samples and each LDS byte feeds 64
-// output features.  The next layer's weight slice is prefetched into registers
-// while the current layer runs.  Outputs (fp32 accumulators + bias +
-// activation) are written as bf16 into the other LDS buffer in the B-operand
-// layout; one barrier per layer.  The encodings live in LDS too; the sigma
-// and rgb heads are fp32 dot products over the bf16 activations.
+// For each layer, wave w owns output blocks {2w, 2w+1} (64 features) and holds
+// that slice of the packed bf16 weights in VGPRs, split into two halves of the
+// layer's k-steps (wlo, whi).  The layer runs k-half by k-half over all four
+// sample blocks, with the accumulators of all four blocks live (128 AGPRs):
+// once the first half has been used, the next layer's first half is fetched
+// into wlo while the second half runs; once the second half is done, the next
+// layer's second half is fetched into whi while the next layer's first half
+// runs.  Every weight byte fetched from L2 feeds 128 samples, every LDS byte
+// feeds 64 output features, and no more than one layer's slice is resident.
+//
+// B operands (activations, bf16, in the MFMA B layout) stream from an LDS
+// buffer through a ring of four registers (three loads in flight).  The
+// accumulators start from the bias (the first MFMA's C operand is the bias in
+// accumulator layout), so the epilogue of a sample block is: bf16 conversion,
+// ReLU as a packed integer max on the bf16 bits (equal to ReLU before the
+// rounding, which keeps the sign), two LDS stores and, in training, the stash
+// rows.  Each block's epilogue is issued right after its last MFMA, so it
+// overlaps the MFMAs of the next block.  Out-of-range samples of the last
+// workgroup compute the values of the last sample and write them to its rows
+// (identical bytes), so the epilogue has no per-lane branch.  One barrier per
+// layer; the sigma head and the rgb branch follow the reference's topology.
 #pragma once
+#include <type_traits>
+
 #include "mlp_kernels.h"
 
 namespace ws {
@@ -21,21 +35,107 @@ constexpr int kT = 16;           // bf16 k-steps over a 256-wide hidden input
 constexpr int kTX = 4;           // k-steps over the xyz encoding slots (32 per half)
 constexpr int kTD = 2;           // k-steps over the direction encoding slots (16 per half)
 constexpr int kSlab = kT * 64;   // bf16x8 per sample block in one activation buffer
+constexpr int kHalf = 10;        // register k-steps per half (20-step skip layer)
 
-// Load the first TL k-steps of this wave's weight slice (output blocks nb0,
-// nb0+1): w[t][i] = A operand of k-step t, output block nb0 + i.
-template <int TL, int TT>
-__device__ __forceinline__ void load_slice(bf16x8 (&w)[TT][2], const bf16x8* __restrict__ wq, int NBo,
-                                           int nb0, int lane) {
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// The packed weights as a buffer resource: every weight / bias load is a
+// buffer load with one shared per-lane VGPR offset and a wave-uniform SGPR
+// offset, so no per-lane 64-bit address is ever computed or kept live.  Loads
+// past packed_bytes return zeros instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t packed_rsrc(const char* pk, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(pk), 0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ bf16x8 ldw(__amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
+  return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ floatx4 ldf4(__amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
+  return __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, (int)soff, 0));
+}
+
+// k-steps OFF .. OFF+N-1 of output blocks nb0, nb0 + 1 of the packed layer at
+// byte offset p_layer into w[0 .. N); lane_off = 16 * lane.
+template <int N, int OFF>
+__device__ __forceinline__ void load_w(bf16x8 (&w)[kHalf][2], __amdgpu_buffer_rsrc_t r, int64_t p_layer, int NBo,
+                                       int nb0, uint32_t lane_off) {
 #pragma unroll
-  for (int t = 0; t < TL; ++t)
+  for (int t = 0; t < N; ++t)
 #pragma unroll
-    for (int i = 0; i < 2; ++i) w[t][i] = wq[(t * NBo + nb0 + i) * 64 + lane];
+    for (int i = 0; i < 2; ++i) w[t][i] = ldw(r, lane_off, p_layer + (int64_t)((OFF + t) * NBo + nb0 + i) * 1024);
+}
+
+// Bias of output block nb in accumulator layout: element 4a + r of lane
+// (j, g) holds feature 32 nb + 8 a + 4 g + r (g_off = 16 * g bytes).
+__device__ __forceinline__ floatx16 bias_acc(__amdgpu_buffer_rsrc_t rs, int64_t p_bias, int nb, uint32_t g_off) {
+  floatx16 c;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const floatx4 v = ldf4(rs, g_off, p_bias + 4 * (32 * nb + 8 * a));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c[4 * a + r] = v[r];
+  }
+  return c;
+}
+
+__device__ __forceinline__ bf16x8 relu_bf16(bf16x8 v) {
+  const shortx8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+  return __builtin_bit_cast(bf16x8, __builtin_elementwise_max(__builtin_bit_cast(shortx8, v), z));
+}
+
+typedef __attribute__((address_space(3))) bf16x8 lds_b8;
+
+// Per-lane LDS pointer to element `lane` (+ off) of a bf16x8 array in LDS,
+// opaque to the optimiser, so that every access through it is this one VGPR
+// plus an immediate offset (all offsets used below are < 64 KB).
+__device__ __forceinline__ lds_b8* lds_lane(const bf16x8* arr, int off, int lane) {
+  uint32_t a = (uint32_t)(uintptr_t)(const lds_b8*)arr + 16u * (uint32_t)(off + lane);
+  asm volatile("" : "+v"(a));
+  return (lds_b8*)(uintptr_t)a;
+}
+
+// B operand of k-step t (0 <= t < 20) of sample block sb: hidden features from
+// `in` for t < 16, the xyz encoding for t >= 16 (per-lane pointers).
+__device__ __forceinline__ bf16x8 b_operand(const lds_b8* in, const lds_b8* ex, int sb, int t) {
+  return t < kT ? in[sb * kSlab + t * 64] : ex[(sb * kTX + (t - kT)) * 64];
+}
+
+// acc[sb][i] (+)= sum over k-steps T0 .. T0+H-1 of w[k - T0][i] * B(sb, k), for
+// all four sample blocks.  INIT: the first k-step starts from the bias cb.
+// After the last k-step of each block, epi(sb) runs (no-op for a first half).
+template <int T0, int H, bool INIT, typename Epi>
+__device__ __forceinline__ void run_steps(floatx16 (&acc)[kBlocks][2], const bf16x8 (&w)[kHalf][2],
+                                          const floatx16 (&cb)[2], const lds_b8* in, const lds_b8* ex,
+                                          Epi&& epi) {
+  constexpr int S = kBlocks * H;
+  bf16x8 b[4];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) b[s] = b_operand(in, ex, s / H, T0 + s % H);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (s + 3 < S) b[(s + 3) & 3] = b_operand(in, ex, (s + 3) / H, T0 + (s + 3) % H);
+    const int sb = s / H, k = s % H;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      acc[sb][i] = mfma_bf16(w[k][i], b[s & 3], (INIT && k == 0) ? cb[i] : acc[sb][i]);
+    if (k == H - 1) epi(sb);
+  }
+}
+
+template <int A, int B, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (A < B) {
+    f(std::integral_constant<int, A>{});
+    static_for<A + 1, B>(f);
+  }
 }
 
 }  // namespace ws
 
-template <int W, int D, int SKIP, int LX, int LD>
+template <int W, int D, int SKIP, int LX, int LD, bool STASH>
 __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
     MlpPlan P, const char* __restrict__ pk, int64_t M, const float* __restrict__ xyz,
     const float* __restrict__ dirs, const float* __restrict__ ro, const float* __restrict__ rd,
@@ -43,13 +143,49 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
     float* __restrict__ sig_out, __bf16* __restrict__ stash) {
   using namespace ws;
   using C = NetC<W, D, SKIP, LX, LD>;
-  static_assert(W == 256 && LX == 10 && LD == 4, "weight-stationary kernel is for the 8x256 NeRF MLP");
+  static_assert(W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D,
+                "weight-stationary kernel is for the 8x256 NeRF MLP");
   __shared__ bf16x8 s_act[2][kBlocks * kSlab];          // 2 x 64 KB
   __shared__ bf16x8 s_ex[kBlocks * kTX * 64];           // 16 KB
   __shared__ bf16x8 s_ed[kBlocks * kTD * 64];           // 8 KB
   __shared__ float s_red[kWaves][3][kBlocks * 32];     // rgb-head partials, 6 KB
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, j = lane & 31, g = lane >> 5;
+  const int lane = threadIdx.x & 63, j = lane & 31, g = lane >> 5;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int64_t base = (int64_t)blockIdx.x * (kBlocks * 32);
+  const int nb0 = 2 * wv;  // this wave's output blocks of a 256-wide layer
+  const uint32_t lane_off = 16u * (uint32_t)lane, g_off = 16u * (uint32_t)g;
+  const __amdgpu_buffer_rsrc_t rs = packed_rsrc(pk, P.packed_bytes);
+  // stash row of sample block sb (clamped: out-of-range lanes rewrite the last row)
+  auto row = [&](int sb) {
+    const int64_t m = base + sb * 32 + j;
+    return stash + (m < M ? m : M - 1) * P.stash_per_sample;
+  };
+  // first / second half of a layer's k-steps into wlo / whi
+  auto load_half = [&](bf16x8 (&w)[kHalf][2], int m, bool enc, bool second) {
+    if (enc) {
+      constexpr int H = (kT + kTX) / 2;
+      if (second)
+        load_w<H, H>(w, rs, P.p_fwd[m], C::NB, nb0, lane_off);
+      else
+        load_w<H, 0>(w, rs, P.p_fwd[m], C::NB, nb0, lane_off);
+    } else {
+      constexpr int H = kT / 2;
+      if (second)
+        load_w<H, H>(w, rs, P.p_fwd[m], C::NB, nb0, lane_off);
+      else
+        load_w<H, 0>(w, rs, P.p_fwd[m], C::NB, nb0, lane_off);
+    }
+  };
+
+  bf16x8 wlo[kHalf][2], whi[kHalf][2];
+  floatx16 cb[2];
+  floatx16 acc[kBlocks][2];
+
+  // layer 0 weights (4 encoding k-steps) into whi, layer 1's first half into wlo
+  load_w<kTX, 0>(whi, rs, P.p_fwd[0], C::NB, nb0, lane_off);
+  load_half(wlo, 1, SKIP == 0, false);
+  cb[0] = bias_acc(rs, P.p_bias[0], nb0, g_off);
+  cb[1] = bias_acc(rs, P.p_bias[0], nb0 + 1, g_off);
 
   // ---- encodings of this wave's own sample block (wave w encodes block w)
   {
@@ -67,177 +203,128 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
     for (int t = 0; t < kTX; ++t) s_ex[(wv * kTX + t) * 64 + lane] = exb[t];
 #pragma unroll
     for (int t = 0; t < kTD; ++t) s_ed[(wv * kTD + t) * 64 + lane] = edb[t];
-    if (stash && m < M) {
-      store_slots_bf16<C::KX>(ex, stash + m * P.stash_per_sample + P.st_encx + g * C::KX);
-      store_slots_bf16<C::KD>(ed, stash + m * P.stash_per_sample + P.st_encd + g * C::KD);
+    if (STASH) {
+      __bf16* r = stash + mm * P.stash_per_sample;
+      store_slots_bf16<C::KX>(ex, r + P.st_encx + g * C::KX);
+      store_slots_bf16<C::KD>(ed, r + P.st_encd + g * C::KD);
     }
   }
-  const int nb0 = 2 * wv;  // this wave's output blocks of a 256-wide layer
 
-  // Runs one MFMA layer for all sample blocks.  `in` (or the encoding for layer
-  // 0) provides T_h hidden k-steps, followed by T_e encoding k-steps from `enc`.
-  // Result: bias + activation, bf16 into `out` (B-operand layout) and the stash.
-  auto run_layer = [&](auto w_slice, auto t_h_c, auto t_e_c, const bf16x8* in, const bf16x8* enc,
-                       int enc_T, bf16x8* out, const float* bias, int relu, int64_t st_off) {
-    constexpr int T_h = decltype(t_h_c)::value;
-    constexpr int T_e = decltype(t_e_c)::value;
-    auto& w = *w_slice;
-#pragma unroll 1
-    for (int sb = 0; sb < kBlocks; ++sb) {
-      floatx16 acc[2];
+  // Epilogue of sample block sb of a 256-wide layer: bf16, ReLU, LDS (B layout), stash.
+  // out: per-lane pointer at k-step 2 nb0 of sample block 0 of the output buffer
+  auto epilogue = [&](lds_b8* out, bool relu, int64_t st_off, int sb) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
-#pragma unroll
-      for (int t = 0; t < T_h; ++t) {
-        const bf16x8 b = in[sb * kSlab + t * 64 + lane];
-        acc[0] = mfma_bf16(w[t][0], b, acc[0]);
-        acc[1] = mfma_bf16(w[t][1], b, acc[1]);
+    for (int i = 0; i < 2; ++i) {
+      bf16x8 lo = acc_to_bf16(acc[sb][i], 0), hi = acc_to_bf16(acc[sb][i], 1);
+      if (relu) {
+        lo = relu_bf16(lo);
+        hi = relu_bf16(hi);
       }
-#pragma unroll
-      for (int t = 0; t < T_e; ++t) {
-        const bf16x8 b = enc[(sb * enc_T + t) * 64 + lane];
-        acc[0] = mfma_bf16(w[T_h + t][0], b, acc[0]);
-        acc[1] = mfma_bf16(w[T_h + t][1], b, acc[1]);
-      }
-      // bias + activation on the two output blocks nb0, nb0+1
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const floatx4 bv = *reinterpret_cast<const floatx4*>(bias + 32 * (nb0 + i) + 8 * a + 4 * g);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            float v = acc[i][4 * a + r] + bv[r];
-            if (relu) v = v > 0.f ? v : 0.f;
-            acc[i][4 * a + r] = v;
-          }
-        }
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        out[sb * kSlab + (2 * (nb0 + i)) * 64 + lane] = acc_to_bf16(acc[i], 0);
-        out[sb * kSlab + (2 * (nb0 + i) + 1) * 64 + lane] = acc_to_bf16(acc[i], 1);
-      }
-      if (stash && st_off >= 0) {
-        const int64_t m = base + sb * 32 + j;
-        if (m < M) {
-          __bf16* row = stash + m * P.stash_per_sample + st_off + 32 * nb0;
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-              const floatx4_ v = {acc[i][4 * a], acc[i][4 * a + 1], acc[i][4 * a + 2], acc[i][4 * a + 3]};
-              *reinterpret_cast<bf16x4*>(row + 32 * i + 8 * a + 4 * g) = __builtin_convertvector(v, bf16x4);
-            }
-        }
+      out[sb * kSlab + (2 * i) * 64] = lo;
+      out[sb * kSlab + (2 * i + 1) * 64] = hi;
+      if (STASH) {
+        // acc element 4a + r <-> feature 32 (nb0 + i) + 8 a + 4 g + r; lo holds a = 0, 1, hi a = 2, 3
+        __bf16* rr = row(sb) + st_off + 32 * (nb0 + i) + 4 * g;
+        *reinterpret_cast<bf16x4*>(rr + 0) = __builtin_shufflevector(lo, lo, 0, 1, 2, 3);
+        *reinterpret_cast<bf16x4*>(rr + 8) = __builtin_shufflevector(lo, lo, 4, 5, 6, 7);
+        *reinterpret_cast<bf16x4*>(rr + 16) = __builtin_shufflevector(hi, hi, 0, 1, 2, 3);
+        *reinterpret_cast<bf16x4*>(rr + 24) = __builtin_shufflevector(hi, hi, 4, 5, 6, 7);
       }
     }
   };
 
-  using I0 = std::integral_constant<int, 0>;
-  using IT = std::integral_constant<int, kT>;
-  using ITX = std::integral_constant<int, kTX>;
-  using ITD = std::integral_constant<int, kTD>;
-  auto wq = [&](int m) { return reinterpret_cast<const bf16x8*>(pk + P.p_fwd[m]); };
-  auto bias = [&](int m) { return reinterpret_cast<const float*>(pk + P.p_bias[m]); };
-
-  // layer 0 (encoding only) — its slice is small; load it directly
+  __syncthreads();  // encodings visible
+  // ---- layer 0 (encoding only, weights in whi)
+  lds_b8* const ex_l = lds_lane(s_ex, 0, lane);
   {
-    bf16x8 w0[kTX][2];
-    ws::load_slice<kTX, kTX>(w0, wq(0), C::NB, nb0, lane);
-    __syncthreads();  // encodings visible
-    run_layer(&w0, I0{}, ITX{}, nullptr, s_ex, kTX, s_act[0], bias(0), 1, P.st_h[0]);
+    lds_b8* const out_l = lds_lane(s_act[0], 2 * nb0 * 64, lane);
+    run_steps<kT, kTX, true>(acc, whi, cb, nullptr, ex_l, [&](int sb) { epilogue(out_l, true, P.st_h[0], sb); });
   }
-  // hidden layers 1..D-1 and the bottleneck (layer D), weights prefetched one layer ahead
-  bf16x8 wA[kT + kTX][2], wB[kT + kTX][2];
-  if (SKIP == 0)
-    ws::load_slice<kT + kTX, kT + kTX>(wA, wq(1), C::NB, nb0, lane);  // layer 1 with its enc rows
-  else
-    ws::load_slice<kT, kT + kTX>(wA, wq(1), C::NB, nb0, lane);
-  int cur = 0;
-#pragma unroll
-  for (int l = 1; l <= D; ++l) {  // l == D: bottleneck (linear)
-    auto& wcur = (l % 2) ? wA : wB;
-    auto& wnext = (l % 2) ? wB : wA;
-    if (l + 1 <= D) {
-      if (l == SKIP)  // layer l+1 re-reads the encoding
-        ws::load_slice<kT + kTX, kT + kTX>(wnext, wq(l + 1), C::NB, nb0, lane);
-      else
-        ws::load_slice<kT, kT + kTX>(wnext, wq(l + 1), C::NB, nb0, lane);
-    }
-    __syncthreads();  // layer l-1 outputs complete
-    const int64_t st_off = l < D ? P.st_h[l] : P.st_bott;
-    if (l - 1 == SKIP)
-      run_layer(&wcur, IT{}, ITX{}, s_act[cur], s_ex, kTX, s_act[cur ^ 1], bias(l), 1, st_off);
-    else
-      run_layer(&wcur, IT{}, I0{}, s_act[cur], s_ex, kTX, s_act[cur ^ 1], bias(l), l < D, st_off);
-    cur ^= 1;
-    if (l == D - 1) {
-      // sigma head on h_{D-1} (now in s_act[cur]): wave w handles sample block w
-      __syncthreads();
-      const float* hs = reinterpret_cast<const float*>(pk + P.p_head_sigma) + g * C::KH;
+  load_half(whi, 1, SKIP == 0, true);  // layer 1's second half and bias
+  cb[0] = bias_acc(rs, P.p_bias[1], nb0, g_off);
+  cb[1] = bias_acc(rs, P.p_bias[1], nb0 + 1, g_off);
+  __syncthreads();
+
+  // ---- layers 1 .. D (D: the linear bottleneck), unrolled
+  static_for<1, D + 1>([&](auto Lc) {
+    constexpr int l = decltype(Lc)::value;
+    constexpr int cur = (l - 1) & 1;
+    const bf16x8* in = s_act[cur];
+    lds_b8* const in_l = lds_lane(s_act[cur], 0, lane);
+    lds_b8* const out_l = lds_lane(s_act[cur ^ 1], 2 * nb0 * 64, lane);
+    if constexpr (l == D) {
+      // sigma head on h_{D-1} (the input of this layer): wave w handles sample block w
+      const uint32_t hs_off = 4u * (uint32_t)(g * C::KH);  // this lane half's head weights
       float zs = 0.f;
 #pragma unroll
       for (int t = 0; t < kT; ++t) {
-        const bf16x8 hv = s_act[cur][wv * kSlab + t * 64 + lane];
+        const bf16x8 hv = in[wv * kSlab + t * 64 + lane];
         const int nb = t >> 1, h = t & 1;
+        const floatx4 w0 = ldf4(rs, hs_off, P.p_head_sigma + 4 * (16 * nb + 8 * h));
+        const floatx4 w1 = ldf4(rs, hs_off, P.p_head_sigma + 4 * (16 * nb + 8 * h + 4));
 #pragma unroll
-        for (int r = 0; r < 8; ++r) zs += (float)hv[r] * hs[16 * nb + 8 * h + r];
+        for (int r = 0; r < 4; ++r) zs += (float)hv[r] * w0[r];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zs += (float)hv[4 + r] * w1[r];
       }
       zs = zs + shfl_xor(zs, 32);
       zs = zs + reinterpret_cast<const float*>(pk + P.p_head_sigma)[2 * C::KH];
       const int64_t m = base + wv * 32 + j;
       if (m < M && g == 0) sig_out[m] = relu(zs);
     }
-  }
-  // rgb branch dense_{D+1}: 4 output blocks, one per wave; input = bottleneck (s_act[cur]) + dir enc
+    constexpr bool enc = l == SKIP + 1;
+    constexpr int H = enc ? (kT + kTX) / 2 : kT / 2;
+    run_steps<0, H, true>(acc, wlo, cb, in_l, ex_l, [](int) {});
+    // wlo is free: the next layer's first half and bias (after the bottleneck: the
+    // rgb branch's 18 k-steps of output block wv and its bias)
+    if constexpr (l < D) {
+      load_half(wlo, l + 1, l + 1 == SKIP + 1, false);
+      cb[0] = bias_acc(rs, P.p_bias[l + 1], nb0, g_off);
+      cb[1] = bias_acc(rs, P.p_bias[l + 1], nb0 + 1, g_off);
+    } else {
+#pragma unroll
+      for (int t = 0; t < kT + kTD; ++t)
+        wlo[t >> 1][t & 1] = ldw(rs, lane_off, P.p_fwd[D + 1] + (int64_t)(t * C::NBR + wv) * 1024);
+      cb[0] = bias_acc(rs, P.p_bias[D + 1], wv, g_off);
+    }
+    const int64_t st_off = l < D ? P.st_h[l] : P.st_bott;
+    run_steps<H, H, false>(acc, whi, cb, in_l, ex_l, [&](int sb) { epilogue(out_l, l < D, st_off, sb); });
+    if constexpr (l < D) load_half(whi, l + 1, l + 1 == SKIP + 1, true);  // whi is free
+    __syncthreads();  // layer l outputs complete
+  });
+  constexpr int fin = D & 1;  // buffer holding the bottleneck
+
+  // ---- rgb branch dense_{D+1}: one output block per wave (wv), input = bottleneck + dir enc
   {
-    const int mr = D + 1;
-    const bf16x8* wr = reinterpret_cast<const bf16x8*>(pk + P.p_fwd[mr]);
-    bf16x8 w1[kT + kTD];
-#pragma unroll
-    for (int t = 0; t < kT + kTD; ++t) w1[t] = wr[(t * C::NBR + wv) * 64 + lane];
-    __syncthreads();
-    const float* br = reinterpret_cast<const float*>(pk + P.p_bias[mr]);
+    lds_b8* const in_l = lds_lane(s_act[fin], 0, lane);
+    lds_b8* const ed_l = lds_lane(s_ed, 0, lane);
     const floatx4* hr4 = reinterpret_cast<const floatx4*>(pk + P.p_head_rgb);
-#pragma unroll 1
+#pragma unroll
     for (int sb = 0; sb < kBlocks; ++sb) {
-      floatx16 acc;
+      floatx16 a = cb[0];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+      for (int t = 0; t < kT; ++t) a = mfma_bf16(wlo[t >> 1][t & 1], in_l[sb * kSlab + t * 64], a);
 #pragma unroll
-      for (int t = 0; t < kT; ++t) acc = mfma_bf16(w1[t], s_act[cur][sb * kSlab + t * 64 + lane], acc);
+      for (int t = 0; t < kTD; ++t)
+        a = mfma_bf16(wlo[(kT + t) >> 1][(kT + t) & 1], ed_l[(sb * kTD + t) * 64], a);
+      // ReLU, rounded to bf16 (the stashed activation), then the fp32 head partials
 #pragma unroll
-      for (int t = 0; t < kTD; ++t) acc = mfma_bf16(w1[kT + t], s_ed[(sb * kTD + t) * 64 + lane], acc);
+      for (int r = 0; r < 16; ++r) a[r] = (float)(__bf16)(a[r] > 0.f ? a[r] : 0.f);
+      if (STASH) {
+        __bf16* rr = row(sb) + P.st_rbr + 32 * wv + 4 * g;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const floatx4 bv = *reinterpret_cast<const floatx4*>(br + 32 * wv + 8 * a + 4 * g);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float v = acc[4 * a + r] + bv[r];
-          acc[4 * a + r] = (float)(__bf16)(v > 0.f ? v : 0.f);
+        for (int q = 0; q < 4; ++q) {
+          const floatx4_ v = {a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]};
+          *reinterpret_cast<bf16x4*>(rr + 8 * q) = __builtin_convertvector(v, bf16x4);
         }
       }
-      if (stash) {
-        const int64_t m = base + sb * 32 + j;
-        if (m < M) {
-          __bf16* row = stash + m * P.stash_per_sample + P.st_rbr + 32 * wv;
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            const floatx4_ v = {acc[4 * a], acc[4 * a + 1], acc[4 * a + 2], acc[4 * a + 3]};
-            *reinterpret_cast<bf16x4*>(row + 8 * a + 4 * g) = __builtin_convertvector(v, bf16x4);
-          }
-        }
-      }
-      // partial rgb dot over this wave's 32 features of the rgb branch
       float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const floatx4 w = hr4[g * 16 * C::NBR + 16 * wv + i];
-        p0 += acc[i] * w[0];
-        p1 += acc[i] * w[1];
-        p2 += acc[i] * w[2];
+        p0 += a[i] * w[0];
+        p1 += a[i] * w[1];
+        p2 += a[i] * w[2];
       }
       p0 += shfl_xor(p0, 32);
       p1 += shfl_xor(p1, 32);
@@ -268,8 +355,12 @@ int launch_fwd_bf16ws(const MlpPlan& P, const void* packed, int64_t M, const flo
                       const float* dirs, const float* ro, const float* rd, const float* tv, int spr,
                       float* rgb, float* sigma, void* stash, hipStream_t s) {
   const int64_t nwg = (M + 127) / 128;
-  mlp_fwd_bf16ws_kernel<W, D, SKIP, LX, LD><<<(unsigned)nwg, 256, 0, s>>>(
-      P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma,
-      reinterpret_cast<__bf16*>(stash));
+  if (stash)
+    mlp_fwd_bf16ws_kernel<W, D, SKIP, LX, LD, true><<<(unsigned)nwg, 256, 0, s>>>(
+        P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma,
+        reinterpret_cast<__bf16*>(stash));
+  else
+    mlp_fwd_bf16ws_kernel<W, D, SKIP, LX, LD, false><<<(unsigned)nwg, 256, 0, s>>>(
+        P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma, nullptr);
   return 0;
 }

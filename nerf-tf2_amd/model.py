@@ -276,74 +276,98 @@ def exponential_decay(step: int, initial=5e-4, decay_steps=500000, decay_rate=0.
 
 
 class AdamTF:
-    """Keras Adam (beta1 0.9, beta2 0.999, epsilon 1e-7) with TF's ApplyAdam update,
-    one fused kernel launch per parameter vector.  The per-step coefficients are
-    computed on the host in float32 like TF's _prepare_local."""
+    """Keras Adam (beta1 0.9, beta2 0.999, epsilon 1e-7) with TF's ApplyAdam update
+    and ExponentialDecay(5e-4, 500000, 0.1), one fused kernel launch per parameter
+    vector.  The iteration count and the per-step coefficients (TF's
+    _prepare_local, float32 operands) live on the device in a nerf_step_state, so
+    a step needs nothing from the host and can be captured in a HIP graph; the
+    same state carries the loss scale of the BF16 form (DynamicLossScaler)."""
 
-    def __init__(self, models: Sequence[NerfMLP], b1=0.9, b2=0.999, eps=1e-7, schedule=exponential_decay):
+    _NEVER = 2 ** 31 - 1
+
+    def __init__(self, models: Sequence[NerfMLP], b1=0.9, b2=0.999, eps=1e-7, initial_lr=5e-4,
+                 decay_steps=500000, decay_rate=0.1):
         self.models = list(models)
+        dev = self.models[0].device
         self.m = [torch.zeros_like(mm.params) for mm in self.models]
         self.v = [torch.zeros_like(mm.params) for mm in self.models]
-        self.iterations = 0
         self.b1, self.b2, self.eps = np.float32(b1), np.float32(b2), np.float32(eps)
-        self.schedule = schedule
+        self.initial_lr, self.decay_steps, self.decay_rate = float(initial_lr), int(decay_steps), float(decay_rate)
+        self.state = torch.zeros(16, dtype=torch.int32, device=dev)   # 64-byte nerf_step_state
+        self.state.view(torch.float32)[0] = 1.0                        # loss scale
 
-    def coefficients(self):
-        lr = np.float32(self.schedule(self.iterations))
-        t = np.float32(self.iterations + 1)
+    # host reads / writes of the device state (they synchronise: tests and reports)
+    @property
+    def iterations(self) -> int:
+        return int(self.state[4])
+
+    @iterations.setter
+    def iterations(self, it: int) -> None:
+        self.state[4] = int(it)
+
+    def schedule(self, step: int) -> float:
+        return exponential_decay(step, self.initial_lr, self.decay_steps, self.decay_rate)
+
+    def coefficients(self, it: Optional[int] = None):
+        """Host evaluation of (lr_t, 1-b1, 1-b2) at iteration `it`; the device computes the same."""
+        it = self.iterations if it is None else it
+        lr = np.float32(self.schedule(it))
+        t = np.float32(it + 1)
         one = np.float32(1.0)
         b1p = _pow_f32(self.b1, t)
         b2p = _pow_f32(self.b2, t)
         lr_t = np.float32(np.float32(lr * np.sqrt(np.float32(one - b2p), dtype=np.float32)) / np.float32(one - b1p))
         return float(lr_t), float(np.float32(one - self.b1)), float(np.float32(one - self.b2))
 
-    def apply_gradients(self, grads: Sequence[Tensor], inv_scale: Optional[Tensor] = None,
-                        skip_flag: Optional[Tensor] = None) -> None:
-        lr_t, omb1, omb2 = self.coefficients()
+    def apply_gradients(self, grads: Sequence[Tensor], scaler: Optional["DynamicLossScaler"] = None) -> None:
+        """One Adam step.  With a scaler the gradients are those of the scaled loss:
+        they are checked for inf/NaN and unscaled, the step is skipped on overflow
+        and the scale is updated, all on the device."""
+        st, dev = ptr(self.state), stream(self.state.device)
+        call("nerf_step_begin", st, self.initial_lr, self.decay_steps, self.decay_rate, float(self.b1),
+             float(self.b2), dev)
+        if scaler is not None:
+            for g in grads:
+                call("nerf_check_finite", g.numel(), ptr(g), st + 8, dev)   # &state->overflow
         for mm, m, v, g in zip(self.models, self.m, self.v, grads):
-            call("nerf_adam_tf", mm.n_params, ptr(mm.params), ptr(m), ptr(v), ptr(g.contiguous()),
-                 lr_t, omb1, omb2, float(self.eps), ptr(inv_scale), ptr(skip_flag), stream(mm.device))
+            call("nerf_adam_tf_state", mm.n_params, ptr(mm.params), ptr(m), ptr(v), ptr(g.contiguous()),
+                 float(self.eps), st, stream(mm.device))
             mm.repack()
-        self.iterations += 1
+        if scaler is None:
+            call("nerf_step_end", st, self._NEVER, 1.0, 1.0, dev)
+        else:
+            call("nerf_step_end", st, scaler.interval, scaler.factor, scaler.min_scale, dev)
 
 
 # ---------------------------------------------------------------------------
 # Dynamic loss scaling (bf16 training)
 # ---------------------------------------------------------------------------
 class DynamicLossScaler:
-    """Scales the loss by S before the backward, checks every gradient for
-    inf/NaN on the device, lets Adam unscale by 1/S (device scalar) and skip the
-    step on overflow; S halves on overflow and doubles after `interval` clean
+    """Dynamic loss scaling held on the device, in the optimizer's
+    nerf_step_state: the loss is multiplied by the device scalar S before the
+    backward; the optimizer checks every gradient for inf/NaN, unscales by 1/S
+    and skips the step on overflow (a skipped step does not advance its
+    iteration count); S halves on overflow and doubles after `interval` clean
     steps."""
 
-    def __init__(self, device, init_scale=2.0 ** 15, interval=2000, factor=2.0, min_scale=1.0):
-        self.scale = float(init_scale)
-        self.interval, self.factor, self.min_scale = interval, factor, min_scale
-        self.good = 0
-        self.skipped = 0
-        self.flag = torch.zeros(1, dtype=torch.int32, device=device)
-        self.inv = torch.ones(1, dtype=torch.float32, device=device)
+    def __init__(self, optimizer: AdamTF, init_scale=2.0 ** 15, interval=2000, factor=2.0, min_scale=1.0):
+        self.optimizer = optimizer
+        self.interval, self.factor, self.min_scale = int(interval), float(factor), float(min_scale)
+        optimizer.state.view(torch.float32)[0] = float(init_scale)
 
-    def prepare(self, grads: Sequence[Tensor]):
-        self.flag.zero_()
-        self.inv.fill_(1.0 / self.scale)
-        for g in grads:
-            call("nerf_check_finite", g.numel(), ptr(g), ptr(self.flag), stream(g.device))
-        return self.inv, self.flag
+    @property
+    def scale_tensor(self) -> Tensor:
+        """The current scale S as a 0-d device tensor (multiply the loss by it)."""
+        return self.optimizer.state.view(torch.float32)[0]
 
-    def update(self) -> bool:
-        """Host-side bookkeeping after the step (reads the overflow flag)."""
-        overflow = bool(int(self.flag.item()))
-        if overflow:
-            self.scale = max(self.scale / self.factor, self.min_scale)
-            self.good = 0
-            self.skipped += 1
-        else:
-            self.good += 1
-            if self.good >= self.interval:
-                self.scale *= self.factor
-                self.good = 0
-        return overflow
+    # host reads (they synchronise: reports only)
+    @property
+    def scale(self) -> float:
+        return float(self.scale_tensor)
+
+    @property
+    def skipped(self) -> int:
+        return int(self.optimizer.state[5])
 
 
 # ---------------------------------------------------------------------------
@@ -383,7 +407,7 @@ class NeRF:
         self.coarse_model = NerfMLP("coarse", mlp_config, precision, seed=seed, device=device)
         self.fine_model = NerfMLP("fine", mlp_config, precision, seed=seed + 1, device=device)
         self.optimizer = AdamTF([self.coarse_model, self.fine_model])
-        self.scaler = DynamicLossScaler(self.device) if precision == "bf16" else None
+        self.scaler = DynamicLossScaler(self.optimizer) if precision == "bf16" else None
 
     @property
     def models(self):
@@ -418,14 +442,11 @@ class NeRF:
                 mdl.params.requires_grad_(False)
             self.optimizer.apply_gradients(grads)
         else:
-            (total * self.scaler.scale).backward()
+            (total * self.scaler.scale_tensor).backward()
             grads = [mdl.params.grad for mdl in models]
             for mdl in models:
                 mdl.params.requires_grad_(False)
-            inv, flag = self.scaler.prepare(grads)
-            self.optimizer.apply_gradients(grads, inv_scale=inv, skip_flag=flag)
-            if self.scaler.update():
-                self.optimizer.iterations -= 1  # a skipped step does not advance Adam
+            self.optimizer.apply_gradients(grads, self.scaler)
 
     def train_step(self, rays_o, rays_d, near, far, rgb, u_coarse=None, u_fine=None) -> Dict[str, Tensor]:
         """model.py:127-180: MSE(coarse) + MSE(fine), backward, Adam step."""

@@ -49,7 +49,7 @@ class _P:
 class ProposalNeRF:
     def __init__(self, params, precision: str = "bf16", seed: int = 0, device="cuda", n_prop: int = 48,
                  fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 1.0,
-                 dilate: int = 1):
+                 dilate: int = 2):
         self.params = params
         self.white_bg = bool(params.system.white_bg)
         self.precision = precision
@@ -65,7 +65,7 @@ class ProposalNeRF:
         self.coarse_model = NerfMLP("coarse", MLPConfig(), precision, seed=seed, device=device)
         self.fine_model = NerfMLP("fine", MLPConfig(), precision, seed=seed + 1, device=device)
         self.optimizer = AdamTF(self.models)
-        self.scaler = DynamicLossScaler(self.device) if precision == "bf16" else None
+        self.scaler = DynamicLossScaler(self.optimizer) if precision == "bf16" else None
         self.last_fallback_fraction = 0.0
 
     @property
@@ -118,22 +118,25 @@ class ProposalNeRF:
         return {k: torch.cat(v) for k, v in outs.items()}
 
     # ------------------------------------------------------------------
-    def train_step(self, rays_o, rays_d, near, far, rgb) -> Dict[str, Tensor]:
+    def train_step(self, rays_o, rays_d, near, far, rgb, u_prop=None, u_fine=None, u_coarse=None) -> Dict[str, Tensor]:
+        """One step; u_prop (N, n_prop), u_fine (N, N_fine) and u_coarse
+        (ceil(N / coarse_stride), N_coarse) inject the uniform draws."""
         for mdl in self.models:
             mdl.params.requires_grad_(True)
             mdl.params.grad = None
         n = rays_o.shape[0]
-        dp, pp_p, df, pp_f = self._proposal_fine(rays_o, rays_d, near, far)
+        dp, pp_p, df, pp_f = self._proposal_fine(rays_o, rays_d, near, far, u_prop, u_fine)
         lf = ray_utils.mse_loss(rgb, pp_f["pred_rgb"])
         # proposal: d(mean interlevel loss)/d w_p from the kernel, applied through a surrogate
         missed, il, g_wp = ray_utils.interlevel(dp["bin_data"]["bin_edges"], pp_p["weights"], df["t_vals"],
                                                 pp_f["weights"], far, want_grad=True,
                                                 grad_scale=self.interlevel_weight / n, dilate=self.dilate)
-        surrogate = (pp_p["weights"] * g_wp).sum()
+        surrogate = ray_utils.inject_grad(pp_p["weights"], g_wp)
         # coarse (fallback) model on a strided subset of the batch
         sub = slice(0, n, self.coarse_stride)
         o, d, nr, fr = rays_o[sub].contiguous(), rays_d[sub].contiguous(), near[sub].contiguous(), far[sub].contiguous()
-        dc = ray_utils.create_input_batch_coarse_model(self.params, o, d, nr, fr, materialize=False)
+        dc = ray_utils.create_input_batch_coarse_model(self.params, o, d, nr, fr, u_vals=u_coarse,
+                                                       materialize=False)
         rgb_c, sig_c = self.coarse_model.forward_rays(o, d, dc["t_vals"])
         pp_c = ray_utils.post_process_model_output(rgb_c, sig_c, dc["t_vals"], self.white_bg)
         lc = ray_utils.mse_loss(rgb[sub].contiguous(), pp_c["pred_rgb"])
@@ -146,14 +149,11 @@ class ProposalNeRF:
                 m.params.requires_grad_(False)
             self.optimizer.apply_gradients(grads)
         else:
-            (total * self.scaler.scale).backward()
+            (total * self.scaler.scale_tensor).backward()
             grads = [m.params.grad if m.params.grad is not None else torch.zeros_like(m.params) for m in models]
             for m in models:
                 m.params.requires_grad_(False)
-            inv, flag = self.scaler.prepare(grads)
-            self.optimizer.apply_gradients(grads, inv_scale=inv, skip_flag=flag)
-            if self.scaler.update():
-                self.optimizer.iterations -= 1
+            self.optimizer.apply_gradients(grads, self.scaler)
         return {"fine_loss": lf.detach(), "coarse_loss": lc.detach(), "interlevel": il.mean().detach(),
                 "missed_mean": missed.mean().detach(),
                 "fallback_fraction": (missed > self.fallback_threshold).float().mean().detach()}

@@ -2,8 +2,8 @@
 
     python -m nerf-tf2_amd.quality --steps 2000          (run from the repo root)
 
-Trains, on identical 4096-ray batches drawn across 100 seed-placed 800x800
-views, (a) the plain FP32 coarse+fine NeRF, (b) the same in BF16 (precision
+Trains (graph-replayed steps), on identical 4096-ray batches drawn across 100
+seed-placed 800x800 views, (a) the plain FP32 coarse+fine NeRF, (b) the same in BF16 (precision
 form, config 2) and (c) the BF16 learned-proposal form (config 4); then
 renders held-out 800x800 views with each and reports per-element PSNR and max
 |rgb error| against the float64 ground truth, the errors of (b) and (c)
@@ -87,11 +87,12 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda", forms_sel=("plain_fp32"
     renders = {}
     for name, make in forms.items():
         net = make()
+        step = nerf.GraphedTrainStep(net, 4096)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(steps):
             o, d, gt = batches[i]
-            st = net.train_step(o, d, near, far, gt)
+            st = step(o, d, near, far, gt)
             if i % 250 == 0 or i == steps - 1:
                 print(f"quality: {name} step {i} fine_loss {float(st['fine_loss']):.5f}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
@@ -121,7 +122,7 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda", forms_sel=("plain_fp32"
         if getattr(net, "scaler", None) is not None:
             res[name]["loss_scale"] = net.scaler.scale
             res[name]["skipped_steps"] = net.scaler.skipped
-        del net
+        del step, net
         torch.cuda.empty_cache()
     for name in ("plain_bf16", "learned_bf16"):
         if name not in res or "plain_fp32" not in res:

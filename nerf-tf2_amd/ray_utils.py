@@ -227,6 +227,26 @@ def mse_loss(y_true: Tensor, y_pred: Tensor) -> Tensor:
     return MSEFunction.apply(y_pred, y_true)
 
 
+class _InjectGrad(torch.autograd.Function):
+    """A zero-valued loss term whose gradient with respect to x is g (times the
+    incoming gradient): applies a gradient computed by a kernel without a
+    reduction kernel in the forward."""
+
+    @staticmethod
+    def forward(ctx, x: Tensor, g: Tensor):
+        ctx.save_for_backward(g)
+        return torch.zeros((), dtype=x.dtype, device=x.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return g * grad_out, None
+
+
+def inject_grad(x: Tensor, g: Tensor) -> Tensor:
+    return _InjectGrad.apply(x, g)
+
+
 def interlevel(prop_edges: Tensor, prop_weights: Tensor, t_fine: Tensor, fine_weights: Tensor, far: Tensor,
                want_grad: bool = False, grad_scale: float = 1.0, dilate: int = 0):
     """Histogram upper-bound loss of the proposal against the fine weights

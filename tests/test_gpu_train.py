@@ -133,6 +133,58 @@ def test_adam_step_matches_oracle(nerf, oracle):
     assert torch.equal(got, var[0]), float((got - var[0]).abs().max())
 
 
+def test_adam_stateless_entry_point(nerf, oracle):
+    """nerf_adam_tf with host coefficients: the same update, bit-exact."""
+    n = 5000
+    g = torch.randn(n)
+    p0 = torch.randn(n)
+    ref = oracle.AdamTF([p0.clone()]).step([p0.clone()], [g])[0]
+    opt = nerf.AdamTF([nerf.NerfMLP("coarse", nerf.MLPConfig(depth=4, width=32, skip=1), "fp32", seed=1)])
+    lr_t, omb1, omb2 = opt.coefficients(0)
+    var, m, v = p0.to(DEV), torch.zeros(n, device=DEV), torch.zeros(n, device=DEV)
+    nerf._lib.call("nerf_adam_tf", n, var.data_ptr(), m.data_ptr(), v.data_ptr(), g.to(DEV).data_ptr(),
+                   lr_t, omb1, omb2, 1e-7, None, None, None)
+    assert torch.equal(var.cpu(), ref)
+
+
+def test_scaled_step_state(nerf, oracle):
+    """Loss-scaled steps on the device state: the unscaled update is bit-exact
+    with the oracle's Adam, an overflowing step is skipped without advancing the
+    iteration count, the scale follows the dynamic rule, and the device
+    coefficients equal the host formula at large iteration counts."""
+    mlp = nerf.NerfMLP("coarse", nerf.MLPConfig(depth=4, width=32, skip=1), "bf16", seed=1)
+    opt = nerf.AdamTF([mlp])
+    scaler = nerf.DynamicLossScaler(opt, init_scale=2.0 ** 10, interval=2, factor=2.0)
+    p0 = mlp.params.detach().cpu().clone()
+    ref_opt = oracle.AdamTF([p0])
+    var = [p0.clone()]
+    scales = []
+    for step in range(5):
+        g = torch.randn(mlp.n_params) * (10.0 ** (step - 2))
+        s = scaler.scale
+        scales.append(s)
+        gs = (g * s).to(DEV)                         # a power-of-two scale: exact
+        if step == 2:
+            gs[7] = float("inf")                      # this step must be skipped
+        opt.apply_gradients([gs], scaler)
+        if step != 2:
+            var = ref_opt.step(var, [g])
+    torch.cuda.synchronize()
+    got = mlp.params.detach().cpu()
+    assert torch.equal(got, var[0]), float((got - var[0]).abs().max())
+    assert scaler.skipped == 1 and opt.iterations == 4
+    # clean, clean (x2 after 2 clean steps), overflow (/2), clean, clean (x2)
+    assert scales == [1024.0, 1024.0, 2048.0, 1024.0, 1024.0] and scaler.scale == 2048.0
+    st = opt.state
+    for it in (0, 1, 17, 999, 123456, 499999, 1000000):
+        opt.iterations = it
+        nerf._lib.call("nerf_step_begin", st.data_ptr(), 5e-4, 500000, 0.1, 0.9, 0.999, None)
+        lr_t, omb1, omb2 = opt.coefficients(it)
+        dev = st.view(torch.float32).cpu().numpy()
+        assert np.float32(dev[6]) == np.float32(lr_t), (it, dev[6], lr_t)
+        assert np.float32(dev[7]) == np.float32(omb1) and np.float32(dev[8]) == np.float32(omb2)
+
+
 def test_train_step_runs_and_learns(nerf):
     """A few steps on one fixed batch of the blob scene: the loss goes down."""
     sc = nerf.scene
