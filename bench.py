@@ -44,6 +44,9 @@ def parse():
                     help="comma list; the first is the headline `value`")
     ap.add_argument("--learned-warmup-steps", type=int, default=2000,
                     help="train steps of the learned form before its render is timed")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="run the train steps eagerly instead of replaying a captured HIP graph "
+                         "(for kernel traces: the profiler does not follow graph replays)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -157,7 +160,7 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
     out = {"train_rays_per_s": None, "train_ms_per_step": None}
     nb = len(batches)
     if args.train_steps > 0 or learned:
-        step = nerf.GraphedTrainStep(net, batches[0][0].shape[0])
+        step = net.train_step if args.no_graphs else nerf.GraphedTrainStep(net, batches[0][0].shape[0])
         pre = args.learned_warmup_steps if learned else 2
         for i in range(pre):
             step(*batches[i % nb])

@@ -155,9 +155,11 @@ typedef struct nerf_mlp_desc {
 int64_t nerf_mlp_param_count(const nerf_mlp_desc* desc);
 /* Bytes of the packed device weights for `precision`. */
 int64_t nerf_mlp_packed_bytes(const nerf_mlp_desc* desc, int32_t precision);
-/* Floats of the training stash per sample (layer inputs saved by the forward). */
+/* Elements of the training stash per sample (layer inputs saved by the
+ * forward): float32 elements for NERF_PREC_F32, bf16 for NERF_PREC_BF16. */
 int64_t nerf_mlp_stash_floats_per_sample(const nerf_mlp_desc* desc);
-/* Floats of the per-sample gradient buffer used by backward / weight_grads. */
+/* Elements of the per-sample gradient buffer (dz) used by backward /
+ * weight_grads: float32 for NERF_PREC_F32, bf16 for NERF_PREC_BF16. */
 int64_t nerf_mlp_dz_floats_per_sample(const nerf_mlp_desc* desc);
 
 /* Pack flat float32 weights into the MFMA operand layout (forward and
@@ -171,8 +173,8 @@ int32_t nerf_mlp_pack(const nerf_mlp_desc* desc, int32_t precision, const float*
  *   rays_o, rays_d (N,3), t_vals (N, samples_per_ray)   with M = N*samples_per_ray,
  *   the sample point being rays_o + t*rays_d (ray_utils.py:251, 389).
  * Outputs rgb (M,3) (NULL allowed for a density-only net) and sigma (M,1).
- * stash: NULL, or M * nerf_mlp_stash_floats_per_sample() floats that the
- * backward reads.  For NERF_PREC_BF16 the stash holds bf16 pairs. */
+ * stash: NULL, or M * nerf_mlp_stash_floats_per_sample() elements (float32,
+ * or bf16 for NERF_PREC_BF16) that the backward reads. */
 int32_t nerf_mlp_forward(const nerf_mlp_desc* desc, int32_t precision, const void* packed,
                          int64_t M, const float* xyz, const float* dirs, const float* rays_o,
                          const float* rays_d, const float* t_vals, int32_t samples_per_ray,
@@ -180,7 +182,8 @@ int32_t nerf_mlp_forward(const nerf_mlp_desc* desc, int32_t precision, const voi
 
 /* Backward through the MLP for M samples: d_rgb (M,3) (NULL for density-only),
  * d_sigma (M,1), the forward's rgb and sigma outputs and stash -> dz, the
- * per-layer pre-activation gradients (M * nerf_mlp_dz_floats_per_sample()). */
+ * per-layer pre-activation gradients (M * nerf_mlp_dz_floats_per_sample()
+ * elements, float32 or, for NERF_PREC_BF16, bf16). */
 int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precision, const void* packed,
                           int64_t M, const float* d_rgb, const float* d_sigma, const float* rgb,
                           const float* sigma, const void* stash, void* dz, void* stream);

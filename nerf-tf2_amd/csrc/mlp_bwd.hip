@@ -12,8 +12,11 @@
 //   as a split-K MFMA GEMM whose partial tiles are summed in a fixed order.
 #include "common.h"
 #include "mlp_kernels.h"
+#include "mlp_bwd_bf16ws.h"
 
 namespace {
+
+typedef unsigned int u32x4_ __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // dX chain
@@ -138,13 +141,13 @@ __device__ __forceinline__ void relu_mask_rows_bf16(floatx16 (&acc)[NBo], const 
     }
 }
 
-// bf16 twin of mlp_bwd_f32_kernel: dz is rounded to bf16 only as the B operand of
-// the next MFMA; the rows written for the weight GEMMs stay fp32.
+// bf16 twin of mlp_bwd_f32_kernel: dz is rounded to bf16 as the B operand of the
+// next MFMA and in the rows written for the weight GEMMs (bf16 dz rows).
 template <int W, int D, int SKIP, int LX, int LD>
 __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
     MlpPlan P, const char* __restrict__ pk, int64_t M, const float* __restrict__ d_rgb,
     const float* __restrict__ d_sigma, const float* __restrict__ rgb,
-    const float* __restrict__ sigma, const __bf16* __restrict__ stash, float* __restrict__ dz) {
+    const float* __restrict__ sigma, const __bf16* __restrict__ stash, __bf16* __restrict__ dz) {
   using C = NetC<W, D, SKIP, LX, LD>;
   constexpr int TH = W / 16;
   __shared__ bf16x8 s_act[4 * TH * 64];
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
   const bool valid = m < M;
   const int64_t mm = valid ? m : M - 1;
   const __bf16* srow = stash + mm * P.stash_per_sample;
-  float* zrow = dz + mm * P.dz_per_sample;
+  __bf16* zrow = dz + mm * P.dz_per_sample;
   const float sg = sigma[mm];
   const float ds = valid ? d_sigma[mm] : 0.f;
   const float dzs = C::RGB ? (sg > 0.f ? ds : 0.f) : ds * (1.f - expf(-sg));
@@ -181,11 +184,11 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
       }
     relu_mask_rows_bf16(accr, srow + P.st_rbr, g);
     if (valid) {
-      store_rows_f32(accr, zrow + P.dz_rbr, g);
+      store_rows_bf16(accr, zrow + P.dz_rbr, g);
       if (g == 0) {
-        zrow[P.dz_rgb + 0] = dzr[0];
-        zrow[P.dz_rgb + 1] = dzr[1];
-        zrow[P.dz_rgb + 2] = dzr[2];
+        zrow[P.dz_rgb + 0] = (__bf16)dzr[0];
+        zrow[P.dz_rgb + 1] = (__bf16)dzr[1];
+        zrow[P.dz_rgb + 2] = (__bf16)dzr[2];
       }
     }
     wave_sync();
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
     wave_sync();
     mfma_lds_bf16<C::NB, 2 * C::NBR>(acc, reinterpret_cast<const bf16x8*>(pk + P.p_bwd[D + 1]), act,
                                      lane);
-    if (valid) store_rows_f32(acc, zrow + P.dz_bott, g);
+    if (valid) store_rows_bf16(acc, zrow + P.dz_bott, g);
     wave_sync();
     store_act_bf16(acc, act, lane);
     zero_acc(acc);
@@ -214,9 +217,9 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
         for (int r = 0; r < 4; ++r) acc[nb][4 * a + r] = acc[nb][4 * a + r] + w[r] * dzs;
       }
   }
-  if (valid && g == 0) zrow[P.dz_sigma] = dzs;
+  if (valid && g == 0) zrow[P.dz_sigma] = (__bf16)dzs;
   relu_mask_rows_bf16(acc, srow + P.st_h[D - 1], g);
-  if (valid) store_rows_f32(acc, zrow + P.dz_h[D - 1], g);
+  if (valid) store_rows_bf16(acc, zrow + P.dz_h[D - 1], g);
 #pragma unroll 1
   for (int l = D - 1; l >= 1; --l) {
     wave_sync();
@@ -225,12 +228,12 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
     wave_sync();
     mfma_lds_bf16<C::NB, TH>(acc, reinterpret_cast<const bf16x8*>(pk + P.p_bwd[l]), act, lane);
     relu_mask_rows_bf16(acc, srow + P.st_h[l - 1], g);
-    if (valid) store_rows_f32(acc, zrow + P.dz_h[l - 1], g);
+    if (valid) store_rows_bf16(acc, zrow + P.dz_h[l - 1], g);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Weight-gradient GEMM: part[split][k][n] = sum over the split's samples of
+// fp32 weight-gradient GEMM: part[split][k][n] = sum over the split's samples of
 // X[m][xoff + k] * Z[m][zoff + n], for k < K, n < N.  128x128 tile per block,
 // 4 waves of 64x64 (2x2 blocks of v_mfma_f32_32x32x2_f32), 32-sample LDS chunks.
 // The block with tile_k == 0 also sums the columns of Z (bias gradient).
@@ -238,8 +241,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
 constexpr int kTile = 128, kChunk = 32, kPad = 4;
 
 struct GemmArgs {
-  const void* X;   // float or bf16 rows (x_bf16)
-  int x_bf16;
+  const float* X;
   int64_t ldx, xoff;
   const float* Z;
   int64_t ldz, zoff;
@@ -286,26 +288,12 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
       floatx4 xv = {0.f, 0.f, 0.f, 0.f}, zv = {0.f, 0.f, 0.f, 0.f};
       if (rok) {
         const int64_t xo = mrow * a.ldx + a.xoff + k0 + col;
-        if (a.x_bf16) {
-          const __bf16* xb = reinterpret_cast<const __bf16*>(a.X);
-          if (k0 + col + 3 < a.K) {
-            const bf16x4 v = *reinterpret_cast<const bf16x4*>(xb + xo);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) xv[e] = (float)v[e];
-          } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-              if (k0 + col + e < a.K) xv[e] = (float)xb[xo + e];
-          }
+        if (k0 + col + 3 < a.K) {
+          xv = *reinterpret_cast<const floatx4*>(a.X + xo);
         } else {
-          const float* xf = reinterpret_cast<const float*>(a.X);
-          if (k0 + col + 3 < a.K) {
-            xv = *reinterpret_cast<const floatx4*>(xf + xo);
-          } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-              if (k0 + col + e < a.K) xv[e] = xf[xo + e];
-          }
+          for (int e = 0; e < 4; ++e)
+            if (k0 + col + e < a.K) xv[e] = a.X[xo + e];
         }
         if (n0 + col + 3 < a.N) {
           zv = *reinterpret_cast<const floatx4*>(a.Z + mrow * a.ldz + a.zoff + n0 + col);
@@ -355,134 +343,196 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
   if (do_bias && n0 + tid < a.N) a.bpart[(int64_t)split * a.N + n0 + tid] = bsum;
 }
 
-// bf16 variant for the bf16 stash: X (bf16) and dz (fp32, rounded to bf16 here)
-// are staged transposed, [feature][sample], into LDS as packed sample pairs so
-// each lane's MFMA operand (8 consecutive samples of one feature) is one
-// ds_read_b128; v_mfma_f32_32x32x16_bf16 with fp32 accumulation.  Bias sums use
-// the fp32 dz (16-lane shuffle reduction per feature group).
-constexpr int kPitch16 = kChunk + 8;  // bf16 per LDS row (16 B aligned, staggered banks)
+// ---------------------------------------------------------------------------
+// bf16 weight-gradient GEMM, one pass.  Workgroup s owns the sample range of
+// split s and computes the whole K x N (<= 256 x 256) partial tile, so every X
+// (bf16 stash) and dz (bf16) row is read once per layer.  64-sample chunks are
+// staged transposed, [feature][sample], into a double-buffered LDS tile as
+// packed sample pairs, so each lane's MFMA operand (8 consecutive samples of one
+// feature) is one ds_read_b128; v_mfma_f32_32x32x16_bf16, fp32 accumulation.
+// While chunk c is multiplied, chunk c+1 goes from registers into the other
+// buffer and chunk c+2 is loaded into registers; one barrier per chunk and no
+// branch in the loop.  Loads go through buffer resources whose range ends at
+// the split's last sample, so samples past it read as zero.  Tile split over
+// the 4 waves: with NB >= 4 column blocks, wave w owns column blocks w, w + 4
+// and every row block; otherwise wave w owns row blocks w, w + 4.  Rows k >= K
+// and columns n >= N of the tile hold whatever the neighbouring features are
+// and are not written out.  Bias sums (sum over samples of dz) are taken from
+// the staged bf16 dz.
+// ---------------------------------------------------------------------------
+constexpr int kWC = 64;       // samples per chunk
+constexpr int kWP = kWC + 8;  // bf16 per LDS row (16-byte aligned rows, staggered banks)
+constexpr int kWF = 256;      // max features per tile side
 
-__global__ __launch_bounds__(256) void wgrad_gemm_bf16_kernel(GemmArgs a) {
-  __shared__ __attribute__((aligned(16))) __bf16 Xt[kTile][kPitch16];
-  __shared__ __attribute__((aligned(16))) __bf16 Zt[kTile][kPitch16];
-  // XCD-aware order: workgroup b runs on XCD b % 8, so all tiles of one sample
-  // split (which read the same X / dz rows) share b % 8 and hence one L2.
-  const int T = a.tiles_k * a.tiles_n;
-  const int xcd = blockIdx.x % 8, yy = blockIdx.x / 8;
-  const int tile = yy % T;
-  const int split = (yy / T) * 8 + xcd;
-  if (split >= a.splits) return;  // padding of the split count to a multiple of 8
-  const int tk = tile / a.tiles_n, tn = tile % a.tiles_n;
-  const int k0 = tk * kTile, n0 = tn * kTile;
+// Several layers' GEMMs of the same tile shape go in one launch: workgroup b
+// runs job b / splits on sample split b % splits, so fewer splits per job keep
+// the machine full and the partial tiles (and their reduction) small.
+constexpr int kMaxJobs = 8;
+struct Wgrad16Job {
+  int64_t xoff, zoff;  // X / dz column offsets of the layer
+  int K, N;
+  float* part;   // [splits][K][N]
+  float* bpart;  // [splits][N] or null
+};
+struct Wgrad16Args {
+  const __bf16* X;
+  int64_t ldx;
+  const __bf16* Z;
+  int64_t ldz;
+  int64_t M, m_per_split;
+  int splits, njobs;
+  Wgrad16Job job[kMaxJobs];
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const __bf16* base, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(base), 0, (int)bytes, 0x00020000);
+}
+
+template <int KB, int NB>
+__global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
+  constexpr bool kModeN = NB >= 4;
+  constexpr int RB = kModeN ? KB : (KB >= 8 ? 2 : 1);  // row blocks per wave
+  constexpr int CB = kModeN ? NB / 4 : NB;             // column blocks per wave
+  constexpr int RX = (KB + 1) / 2, RZ = (NB + 1) / 2;  // staging rounds (64 features each)
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kWF][kWP];
+  __shared__ __attribute__((aligned(16))) __bf16 Zt[2][kWF][kWP];
+  const int split = blockIdx.x % a.splits;
+  const Wgrad16Job J = a.job[blockIdx.x / a.splits];
   const int64_t mb = (int64_t)split * a.m_per_split;
   const int64_t me = mb + a.m_per_split < a.M ? mb + a.m_per_split : a.M;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wk = (wv >> 1) * 64, wn = (wv & 1) * 64;
-  const int r = lane & 31, q = lane >> 5;
-  floatx16 acc[2][2];
+  const int64_t rows = me > mb ? me - mb : 0;
+  const int t = threadIdx.x, lane = t & 63, r = lane & 31, q = lane >> 5;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int pair = t & 31, fg0 = t >> 5;  // staging: sample pair, feature group (+ 8 per round)
+  const __amdgpu_buffer_rsrc_t rx = rows_rsrc(a.X + mb * a.ldx, rows * a.ldx * 2);
+  const __amdgpu_buffer_rsrc_t rz = rows_rsrc(a.Z + mb * a.ldz, rows * a.ldz * 2);
+
+  floatx16 acc[RB][CB];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int i = 0; i < RB; ++i)
 #pragma unroll
-    for (int y = 0; y < 2; ++y)
+    for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) acc[x][y][i] = 0.f;
-  // staging: thread -> sample pair sp (rows 2sp, 2sp+1), feature group fg (8 features)
-  const int sp = tid & 15, fg = tid >> 4;
-  const bool do_bias = a.bpart && tk == 0;
-  float bsum[8];
+      for (int e = 0; e < 16; ++e) acc[i][c][e] = 0.f;
+  float bsum[RZ][8];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
-  const __bf16* Xb = reinterpret_cast<const __bf16*>(a.X);
-  for (int64_t c0 = mb; c0 < me; c0 += kChunk) {
-    float xv[2][8], zv[2][8];
+  for (int i = 0; i < RZ; ++i)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int64_t m = c0 + 2 * sp + h;
-      const bool ok = m < me;
-      const int kf = k0 + 8 * fg, nf = n0 + 8 * fg;
+    for (int e = 0; e < 8; ++e) bsum[i][e] = 0.f;
+
+  u32x4_ xr[RX][2], zr[RZ][2];
+  auto load_chunk = [&](int c) {
+    const uint32_t m0 = (uint32_t)(c * kWC + 2 * pair);
+#pragma unroll
+    for (int i = 0; i < RX; ++i)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        xr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
+            rx, (uint32_t)(((m0 + h) * a.ldx + J.xoff + 8 * (fg0 + 8 * i)) * 2), 0, 0);
+#pragma unroll
+    for (int i = 0; i < RZ; ++i)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        zr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
+            rz, (uint32_t)(((m0 + h) * a.ldz + J.zoff + 8 * (fg0 + 8 * i)) * 2), 0, 0);
+  };
+  auto store_chunk = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < RX; ++i) {
+      const bf16x8 v0 = __builtin_bit_cast(bf16x8, xr[i][0]), v1 = __builtin_bit_cast(bf16x8, xr[i][1]);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        *reinterpret_cast<bf16x2*>(&Xt[buf][8 * (fg0 + 8 * i) + e][2 * pair]) = bf16x2{v0[e], v1[e]};
+    }
+#pragma unroll
+    for (int i = 0; i < RZ; ++i) {
+      const bf16x8 v0 = __builtin_bit_cast(bf16x8, zr[i][0]), v1 = __builtin_bit_cast(bf16x8, zr[i][1]);
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        xv[h][e] = 0.f;
-        zv[h][e] = 0.f;
-      }
-      if (ok) {
-        const __bf16* xr = Xb + m * a.ldx + a.xoff;
-        if (kf + 7 < a.K) {
-          const bf16x8 v = *reinterpret_cast<const bf16x8*>(xr + kf);
-#pragma unroll
-          for (int e = 0; e < 8; ++e) xv[h][e] = (float)v[e];
-        } else {
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-            if (kf + e < a.K) xv[h][e] = (float)xr[kf + e];
-        }
-        const float* zr = a.Z + m * a.ldz + a.zoff;
-        if (nf + 7 < a.N) {
-          const floatx4 v0 = *reinterpret_cast<const floatx4*>(zr + nf);
-          const floatx4 v1 = *reinterpret_cast<const floatx4*>(zr + nf + 4);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            zv[h][e] = v0[e];
-            zv[h][4 + e] = v1[e];
-          }
-        } else {
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-            if (nf + e < a.N) zv[h][e] = zr[nf + e];
-        }
+        *reinterpret_cast<bf16x2*>(&Zt[buf][8 * (fg0 + 8 * i) + e][2 * pair]) = bf16x2{v0[e], v1[e]};
+        bsum[i][e] += (float)v0[e] + (float)v1[e];
       }
     }
+  };
+  auto row_block = [&](int i) { return kModeN ? i : (wv + 4 * i < KB ? wv + 4 * i : KB - 1); };
+  auto col_block = [&](int c) { return kModeN ? wv + 4 * c : c; };
+  auto compute = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-      typedef float floatx2 __attribute__((ext_vector_type(2)));
-      const floatx2 xp = {xv[0][e], xv[1][e]};
-      const floatx2 zp = {zv[0][e], zv[1][e]};
-      *reinterpret_cast<bf16x2*>(&Xt[8 * fg + e][2 * sp]) = __builtin_convertvector(xp, bf16x2);
-      *reinterpret_cast<bf16x2*>(&Zt[8 * fg + e][2 * sp]) = __builtin_convertvector(zp, bf16x2);
-      if (do_bias) bsum[e] += zv[0][e] + zv[1][e];
+    for (int s = 0; s < kWC / 16; ++s) {
+      bf16x8 bz[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c)
+        bz[c] = *reinterpret_cast<const bf16x8*>(&Zt[buf][32 * col_block(c) + r][16 * s + 8 * q]);
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&Xt[buf][32 * row_block(i) + r][16 * s + 8 * q]);
+#pragma unroll
+        for (int c = 0; c < CB; ++c) acc[i][c] = mfma_bf16(ax, bz[c], acc[i][c]);
+      }
     }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kChunk / 16; ++s) {
-      bf16x8 av[2], bv[2];
-#pragma unroll
-      for (int x = 0; x < 2; ++x) av[x] = *reinterpret_cast<const bf16x8*>(&Xt[wk + 32 * x + r][16 * s + 8 * q]);
-#pragma unroll
-      for (int y = 0; y < 2; ++y) bv[y] = *reinterpret_cast<const bf16x8*>(&Zt[wn + 32 * y + r][16 * s + 8 * q]);
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = mfma_bf16(av[x], bv[y], acc[x][y]);
-    }
+  };
+
+  const int nchunks = (int)((rows + kWC - 1) / kWC);
+  load_chunk(0);
+  store_chunk(0);
+  load_chunk(1);
+  __syncthreads();
+  for (int c = 0; c < nchunks; ++c) {
+    compute(c & 1);
+    store_chunk((c + 1) & 1);  // chunk c+1 (zeros past the end; never read then)
+    load_chunk(c + 2);
     __syncthreads();
   }
-  float* out = a.part + (int64_t)split * a.K * a.N;
+
+  float* out = J.part + (int64_t)split * J.K * J.N;
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int i = 0; i < RB; ++i) {
+    const int kb = kModeN ? i : wv + 4 * i;
+    if (kb >= KB) continue;
 #pragma unroll
-    for (int y = 0; y < 2; ++y)
+    for (int c = 0; c < CB; ++c) {
+      const int nn = 32 * col_block(c) + r;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int kk = k0 + wk + 32 * x + 8 * (i / 4) + 4 * q + (i % 4);
-        const int nn = n0 + wn + 32 * y + r;
-        if (kk < a.K && nn < a.N) out[(int64_t)kk * a.N + nn] = acc[x][y][i];
-      }
-  if (do_bias) {
-    // reduce over the 16 sample-pair lanes of this feature group (lanes fg*16 .. fg*16+15)
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-#pragma unroll
-      for (int msk = 1; msk < 16; msk <<= 1) bsum[e] += shfl_xor(bsum[e], msk);
-    }
-    if (sp == 0) {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int n = n0 + 8 * fg + e;
-        if (n < a.N) a.bpart[(int64_t)split * a.N + n] = bsum[e];
+      for (int e = 0; e < 16; ++e) {
+        const int kk = 32 * kb + 8 * (e / 4) + 4 * q + (e % 4);
+        if (kk < J.K && nn < J.N) out[(int64_t)kk * J.N + nn] = acc[i][c][e];
       }
     }
   }
+  if (J.bpart) {
+    // reduce over the 32 sample pairs sharing this feature group (one 32-lane half)
+#pragma unroll
+    for (int i = 0; i < RZ; ++i)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+#pragma unroll
+        for (int msk = 1; msk < 32; msk <<= 1) bsum[i][e] += shfl_xor(bsum[i][e], msk);
+      }
+    if (pair == 0) {
+#pragma unroll
+      for (int i = 0; i < RZ; ++i)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int n = 8 * (fg0 + 8 * i) + e;
+          if (n < J.N) J.bpart[(int64_t)split * J.N + n] = bsum[i][e];
+        }
+    }
+  }
+}
+
+int wg16_blocks(int n) { return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8; }
+
+// All jobs of `a` share the tile shape (kb, nb) = wg16_blocks of their 32-feature blocks.
+int launch_wgrad_bf16(const Wgrad16Args& a, int kb, int nb, hipStream_t s) {
+  const unsigned grid = (unsigned)(a.splits * a.njobs);
+#define NERF_WG16(KB_, NB_) \
+  if (kb == KB_ && nb == NB_) { wgrad_bf16_kernel<KB_, NB_><<<grid, 256, 0, s>>>(a); return 0; }
+#define NERF_WG16_ROW(KB_) NERF_WG16(KB_, 1) NERF_WG16(KB_, 2) NERF_WG16(KB_, 4) NERF_WG16(KB_, 8)
+  NERF_WG16_ROW(1) NERF_WG16_ROW(2) NERF_WG16_ROW(4) NERF_WG16_ROW(8)
+#undef NERF_WG16_ROW
+#undef NERF_WG16
+  return -1;
 }
 
 // grad[k_off + row(k) * fan_out + n] (+)= sum_s part[s][k][n] for n < n_real;
@@ -563,9 +613,11 @@ int wgrad_jobs(const MlpPlan& P, WgradJob* jobs) {
   return n;
 }
 
+// Sample splits of the weight-gradient GEMMs: >= 2048 samples per split, at most
+// 256 splits (one workgroup per CU for the single-pass bf16 kernel).
 int64_t splits_for(int64_t M) {
-  int64_t s = (M + 4095) / 4096;  // >= 4096 samples per split
-  if (s > 128) s = 128;
+  int64_t s = (M + 2047) / 2048;
+  if (s > 256) s = 256;
   return s < 1 ? 1 : s;
 }
 
@@ -587,8 +639,19 @@ int launch_bwd_bf16(const MlpPlan& P, const void* packed, int64_t M, const float
   const int64_t nblk = (M + 31) / 32;
   mlp_bwd_bf16_kernel<W, D, SKIP, LX, LD><<<(unsigned)((nblk + 3) / 4), 256, 0, s>>>(
       P, reinterpret_cast<const char*>(packed), M, d_rgb, d_sigma, rgb, sigma,
-      reinterpret_cast<const __bf16*>(stash), reinterpret_cast<float*>(dz));
+      reinterpret_cast<const __bf16*>(stash), reinterpret_cast<__bf16*>(dz));
   return 0;
+}
+
+// bf16: the weight-stationary chain for the reference topology, the per-wave one otherwise.
+template <int W, int D, int SKIP, int LX, int LD>
+int launch_bwd_bf16_any(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,
+                        const float* d_sigma, const float* rgb, const float* sigma, const void* stash,
+                        void* dz, hipStream_t s) {
+  if constexpr (W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D)
+    return launch_bwd_bf16ws<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
+  else
+    return launch_bwd_bf16<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
 }
 
 int dispatch_bwd(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,
@@ -599,8 +662,8 @@ int dispatch_bwd(const MlpPlan& P, const void* packed, int64_t M, const float* d
     if (P.prec == NERF_PREC_F32)                                                              \
       return launch_bwd_f32<W_, D_, S_, LX_, LD_>(P, packed, M, d_rgb, d_sigma, rgb, sigma,   \
                                                   stash, dz, s);                              \
-    return launch_bwd_bf16<W_, D_, S_, LX_, LD_>(P, packed, M, d_rgb, d_sigma, rgb, sigma,    \
-                                                 stash, dz, s);                               \
+    return launch_bwd_bf16_any<W_, D_, S_, LX_, LD_>(P, packed, M, d_rgb, d_sigma, rgb, sigma,\
+                                                     stash, dz, s);                           \
   }
   NERF_MLP_INSTANCES(NERF_BWD_INST)
 #undef NERF_BWD_INST
@@ -626,12 +689,19 @@ extern "C" int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precisio
   return NERF_OK;
 }
 
+// Partial-tile floats of the weight-gradient workspace: room for max(splits,
+// kMaxJobs) tiles of the widest layer (kmax x W) and their bias rows.  A group of
+// n same-shape jobs uses n x max(1, splits / n) <= max(splits, kMaxJobs) tiles.
+static int64_t wgrad_workspace_floats(const MlpPlan& P, int64_t M) {
+  const int64_t s = splits_for(M) > kMaxJobs ? splits_for(M) : kMaxJobs;
+  const int64_t kmax = (P.W > 2 * P.KX ? P.W : 2 * P.KX);
+  return s * (kmax * P.W + P.W);
+}
+
 extern "C" int64_t nerf_mlp_wgrad_workspace_bytes(const nerf_mlp_desc* desc, int64_t M) {
   MlpPlan P;
   if (nerf_plan_from_desc(desc, 0, &P) || M < 0) return -1;
-  const int64_t s = splits_for(M);
-  const int64_t kmax = (P.W > 2 * P.KX ? P.W : 2 * P.KX);
-  return s * (kmax * P.W + P.W) * 4 + 256;
+  return wgrad_workspace_floats(P, M) * 4 + 256;
 }
 
 extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
@@ -648,39 +718,14 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
   const int64_t splits = splits_for(M);
   const int64_t mps = (M + splits - 1) / splits;
   float* part = reinterpret_cast<float*>(workspace);
-  for (int t = 0; t < nj; ++t) {
-    const WgradJob& J = jobs[t];
-    const int64_t kmax = (P.W > 2 * P.KX ? P.W : 2 * P.KX);
-    float* bpart = J.bias ? part + splits * kmax * P.W : nullptr;
-    GemmArgs a;
-    a.X = stash;
-    a.x_bf16 = precision == NERF_PREC_BF16;
-    a.ldx = P.stash_per_sample;
-    a.xoff = J.xoff;
-    a.Z = reinterpret_cast<const float*>(dz);
-    a.ldz = P.dz_per_sample;
-    a.zoff = J.zoff;
-    a.M = M;
-    a.m_per_split = mps;
-    a.K = J.K;
-    a.N = J.N;
-    a.tiles_k = (J.K + kTile - 1) / kTile;
-    a.tiles_n = (J.N + kTile - 1) / kTile;
-    a.splits = (int)splits;
-    a.part = part;
-    a.bpart = bpart;
-    const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * ((splits + 7) / 8) * 8;
-    if (a.x_bf16)
-      wgrad_gemm_bf16_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
-    else
-      wgrad_gemm_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
+  auto reduce = [&](const WgradJob& J, const float* jp, const float* jb, int nsplit) {
     ReduceArgs r;
-    r.part = part;
-    r.bpart = bpart;
+    r.part = jp;
+    r.bpart = jb;
     r.K = J.K;
     r.N = J.N;
     r.n_real = J.n_real;
-    r.splits = (int)splits;
+    r.splits = nsplit;
     r.enc_L = J.enc_L;
     r.enc_KS = J.enc_KS;
     r.row_base = J.row_base;
@@ -693,6 +738,71 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
     int64_t rb = (tot + 255) / 256;
     if (rb > 1024) rb = 1024;
     wgrad_reduce_kernel<<<(unsigned)rb, 256, 0, s>>>(r);
+  };
+  if (precision == NERF_PREC_BF16) {
+    // group the jobs by tile shape; each group is one launch with splits / njobs splits per job
+    bool done[2 * NERF_MAX_LAYERS] = {};
+    for (int t0 = 0; t0 < nj; ++t0) {
+      if (done[t0]) continue;
+      const int kb = wg16_blocks((jobs[t0].K + 31) / 32), nb = wg16_blocks((jobs[t0].N + 31) / 32);
+      NERF_CHECK_ARG(jobs[t0].K <= 256 && jobs[t0].N <= 256, "nerf_mlp_weight_grads: layer too wide");
+      int members[kMaxJobs], n = 0;
+      for (int t = t0; t < nj && n < kMaxJobs; ++t)
+        if (!done[t] && wg16_blocks((jobs[t].K + 31) / 32) == kb && wg16_blocks((jobs[t].N + 31) / 32) == nb) {
+          members[n++] = t;
+          done[t] = true;
+        }
+      const int64_t sg = splits / n > 0 ? splits / n : 1;
+      Wgrad16Args b;
+      b.X = reinterpret_cast<const __bf16*>(stash);
+      b.ldx = P.stash_per_sample;
+      b.Z = reinterpret_cast<const __bf16*>(dz);
+      b.ldz = P.dz_per_sample;
+      b.M = M;
+      b.m_per_split = (M + sg - 1) / sg;
+      b.splits = (int)sg;
+      b.njobs = n;
+      float* wp = part;
+      for (int i = 0; i < n; ++i) {
+        const WgradJob& J = jobs[members[i]];
+        b.job[i].xoff = J.xoff;
+        b.job[i].zoff = J.zoff;
+        b.job[i].K = J.K;
+        b.job[i].N = J.N;
+        b.job[i].part = wp;
+        wp += sg * J.K * J.N;
+        b.job[i].bpart = J.bias ? wp : nullptr;
+        if (J.bias) wp += sg * J.N;
+      }
+      NERF_CHECK_ARG(wp - part <= wgrad_workspace_floats(P, M), "nerf_mlp_weight_grads: workspace layout");
+      launch_wgrad_bf16(b, kb, nb, s);
+      for (int i = 0; i < n; ++i) reduce(jobs[members[i]], b.job[i].part, b.job[i].bpart, (int)sg);
+    }
+  } else {
+    for (int t = 0; t < nj; ++t) {
+      const WgradJob& J = jobs[t];
+      const int64_t kmax = (P.W > 2 * P.KX ? P.W : 2 * P.KX);
+      float* bpart = J.bias ? part + splits * kmax * P.W : nullptr;
+      GemmArgs a;
+      a.X = reinterpret_cast<const float*>(stash);
+      a.ldx = P.stash_per_sample;
+      a.xoff = J.xoff;
+      a.Z = reinterpret_cast<const float*>(dz);
+      a.ldz = P.dz_per_sample;
+      a.zoff = J.zoff;
+      a.M = M;
+      a.m_per_split = mps;
+      a.K = J.K;
+      a.N = J.N;
+      a.tiles_k = (J.K + kTile - 1) / kTile;
+      a.tiles_n = (J.N + kTile - 1) / kTile;
+      a.splits = (int)splits;
+      a.part = part;
+      a.bpart = bpart;
+      const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * ((splits + 7) / 8) * 8;
+      wgrad_gemm_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
+      reduce(J, part, bpart, (int)splits);
+    }
   }
   NERF_LAUNCH_CHECK("nerf_mlp_weight_grads");
   return NERF_OK;

@@ -1,0 +1,230 @@
+// Weight-stationary BF16 dX chain for the reference topology (W = 256): the
+// backward twin of mlp_fwd_bf16ws_kernel, with the same workgroup shape (4
+// waves x 128 samples), weight halves in VGPRs with next-layer prefetch, LDS
+// double buffer of B operands (here the bf16 dz of the previous step) and
+// per-sample-block epilogues that overlap the next block's MFMAs.
+//
+// Per sample the chain is (model.py:355-392 differentiated):
+//   rgb head:    dz_rgb = dy * y (1 - y);  d h9 = W_rgb dz_rgb (fp32), masked by h9 > 0
+//   rgb branch:  d bott = W_{D+1}[:bott] dz_rbr            (bottleneck: linear)
+//   bottleneck:  d h_{D-1} = W_D dz_bott + W_sigma dz_sigma, masked by h_{D-1} > 0
+//   trunk:       d h_{l-1} = W_l[:hidden] dz_l, masked by h_{l-1} > 0, l = D-1 .. 1
+// Each dz is written as a bf16 row (the weight-gradient GEMMs read them) and,
+// rounded the same way, is the next MFMA's B operand.  The sigma-head term is
+// the accumulators' starting value (an outer product of the head weights and
+// dz_sigma).  ReLU masks read the bf16 activations of the training stash; the
+// mask is applied to the packed bf16 result with integer max / min / and.  Rows
+// are read and written as 16-byte chunks of 8 consecutive features, swapped
+// into / out of the accumulator layout across lane halves (swap_halves).
+// Stash reads and dz writes go through buffer resources that end at the last
+// sample, so lanes past it read zeros and their stores are dropped.
+#pragma once
+#include "mlp_fwd_bf16ws.h"
+
+namespace wsb {
+
+using namespace ws;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// x where h > 0 (both bf16, compared as signed 16-bit integers), else +0.
+__device__ __forceinline__ bf16x8 mask_bf16(bf16x8 x, bf16x8 h) {
+  const shortx8 z = {0, 0, 0, 0, 0, 0, 0, 0}, one = {1, 1, 1, 1, 1, 1, 1, 1};
+  const shortx8 t = __builtin_elementwise_min(__builtin_elementwise_max(__builtin_bit_cast(shortx8, h), z), one);
+  return __builtin_bit_cast(bf16x8, __builtin_bit_cast(shortx8, x) & (z - t));
+}
+
+__device__ __forceinline__ void st4(bf16x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, (int)soff, 0);
+}
+
+// A 16-feature group of a row in the accumulator layout: one 16-byte row-chunk
+// load per lane (features 8 g .. 8 g + 7 at byte offset voff + soff), swapped.
+__device__ __forceinline__ bf16x8 ld_group(__amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
+  return swap_halves(__builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, voff, (int)soff, 0)));
+}
+
+// Stores a 16-feature group given in the accumulator layout as row chunks.
+__device__ __forceinline__ void st_group(bf16x8 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, swap_halves(v)), r, voff, (int)soff, 0);
+}
+
+}  // namespace wsb
+
+template <int W, int D, int SKIP, int LX, int LD>
+__global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
+    MlpPlan P, const char* __restrict__ pk, int64_t M, const float* __restrict__ d_rgb,
+    const float* __restrict__ d_sigma, const float* __restrict__ rgb, const float* __restrict__ sigma,
+    const __bf16* __restrict__ stash, __bf16* __restrict__ dz) {
+  using namespace wsb;
+  using C = NetC<W, D, SKIP, LX, LD>;
+  static_assert(W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D,
+                "weight-stationary backward is for the 8x256 NeRF MLP");
+  static_assert(C::NBR == 4, "rgb branch of 128 features");
+  __shared__ bf16x8 s_act[2][kBlocks * kSlab];  // 2 x 64 KB
+  __shared__ float s_dzs[kBlocks * 32];          // dz_sigma of the workgroup's samples
+  const int lane = threadIdx.x & 63, j = lane & 31, g = lane >> 5;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t base = (int64_t)blockIdx.x * (kBlocks * 32);
+  const int nb0 = 2 * wv;
+  const uint32_t lane_off = 16u * (uint32_t)lane;
+  const __amdgpu_buffer_rsrc_t rs = packed_rsrc(pk, P.packed_bytes);
+  const int64_t nrows = M - base < kBlocks * 32 ? M - base : kBlocks * 32;
+  const int64_t sps = P.stash_per_sample, zps = P.dz_per_sample;
+  const __amdgpu_buffer_rsrc_t rst = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<__bf16*>(stash + base * sps), 0, (int)(nrows * sps * 2), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rdz =
+      __builtin_amdgcn_make_buffer_rsrc(dz + base * zps, 0, (int)(nrows * zps * 2), 0x00020000);
+  // per-lane byte offsets of this lane's sample row in sample block sb (+ 8 g
+  // features: the lane's row chunk of each 16-feature group)
+  auto st_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps + 8 * g) * 2); };
+  auto dz_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * zps + 8 * g) * 2); };
+
+  bf16x8 wlo[kHalf][2], whi[kHalf][2];
+  const floatx16 cb_unused[2] = {};
+  floatx16 acc[kBlocks][2];
+
+  // first MFMA step (rgb branch, 8 k-steps over the 128 rgb-branch features): halves of 4
+  constexpr int KR = 2 * C::NBR;
+  load_w<KR / 2, 0>(wlo, rs, P.p_bwd[D + 1], C::NB, nb0, lane_off);
+  load_w<KR / 2, KR / 2>(whi, rs, P.p_bwd[D + 1], C::NB, nb0, lane_off);
+
+  // ---- heads (wave w: sample block w): dz_rgb, dz_sigma, d h9 -> dz_rbr
+  {
+    const int64_t m = base + wv * 32 + j;
+    const bool valid = m < M;
+    const int64_t mm = valid ? m : M - 1;
+    const float sg = sigma[mm];
+    const float ds = valid ? d_sigma[mm] : 0.f;
+    const float dzs = sg > 0.f ? ds : 0.f;
+    float dzr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float y = rgb[3 * mm + c];
+      const float dy = valid ? d_rgb[3 * mm + c] : 0.f;
+      dzr[c] = dy * y * (1.f - y);
+    }
+    if (g == 0) s_dzs[wv * 32 + j] = dzs;
+    const uint32_t zr = dz_row(wv), sr = st_row(wv);
+    if (g == 0) {
+      const bf16x4 hz = {(__bf16)dzs, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f};
+      const bf16x4 rz = {(__bf16)dzr[0], (__bf16)dzr[1], (__bf16)dzr[2], (__bf16)0.f};
+      st4(hz, rdz, zr, P.dz_sigma * 2);
+      st4(rz, rdz, zr, P.dz_rgb * 2);
+    }
+    const floatx4* hr4 = reinterpret_cast<const floatx4*>(pk + P.p_head_rgb) + g * 16 * C::NBR;
+    lds_b8* const out = lds_lane(s_act[0], wv * kSlab, lane);
+#pragma unroll
+    for (int nb = 0; nb < C::NBR; ++nb) {
+      floatx16 a;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const floatx4 w = hr4[16 * nb + i];
+        a[i] = dzr[0] * w[0] + dzr[1] * w[1] + dzr[2] * w[2];
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const bf16x8 v = mask_bf16(acc_to_bf16(a, h), ld_group(rst, sr, (P.st_rbr + 32 * nb + 16 * h) * 2));
+        out[(2 * nb + h) * 64] = v;
+        st_group(v, rdz, zr, (P.dz_rbr + 32 * nb + 16 * h) * 2);
+      }
+    }
+  }
+  __syncthreads();
+
+  // Epilogue of sample block sb: ReLU mask (MASKED) from the stash activations at
+  // feature offset st_off, bf16 into LDS (B layout; out may be null) and the dz row
+  // at dz_off.
+  auto epilogue = [&](auto masked_c, auto lds_c, lds_b8* out, int64_t st_off, int64_t dz_off, int sb) {
+    constexpr bool MASKED = decltype(masked_c)::value, TO_LDS = decltype(lds_c)::value;
+    const uint32_t zr = dz_row(sb), sr = st_row(sb);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int64_t f0 = 32 * (nb0 + i);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        bf16x8 v = acc_to_bf16(acc[sb][i], h);
+        if constexpr (MASKED) v = mask_bf16(v, ld_group(rst, sr, (st_off + f0 + 16 * h) * 2));
+        if constexpr (TO_LDS) out[sb * kSlab + (2 * i + h) * 64] = v;
+        st_group(v, rdz, zr, (dz_off + f0 + 16 * h) * 2);
+      }
+    }
+  };
+  auto none = [](int) {};
+
+  // ---- rgb branch: d bott = W_{D+1} dz_rbr (linear; 8 k-steps, halves of 4)
+  {
+    lds_b8* const in_l = lds_lane(s_act[0], 0, lane);
+    lds_b8* const out_l = lds_lane(s_act[1], 2 * nb0 * 64, lane);
+#pragma unroll
+    for (int sb = 0; sb < kBlocks; ++sb)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[sb][i][e] = 0.f;
+    run_steps<0, KR / 2, false>(acc, wlo, cb_unused, in_l, in_l, none);
+    load_w<kT / 2, 0>(wlo, rs, P.p_bwd[D], C::NB, nb0, lane_off);
+    run_steps<KR / 2, KR / 2, false>(acc, whi, cb_unused, in_l, in_l,
+                                     [&](int sb) { epilogue(std::false_type{}, std::true_type{}, out_l, 0, P.dz_bott, sb); });
+    load_w<kT / 2, kT / 2>(whi, rs, P.p_bwd[D], C::NB, nb0, lane_off);
+    __syncthreads();
+  }
+
+  // ---- bottleneck and trunk, unrolled: step s = 0 is W_D (input dz_bott), then W_{D-1} .. W_1.
+  static_for<0, D>([&](auto Sc) {
+    constexpr int s = decltype(Sc)::value;
+    constexpr int layer = D - s;          // packed layer whose transpose is applied
+    constexpr int cur = (s + 1) & 1;      // buffer holding this step's input dz
+    lds_b8* const in_l = lds_lane(s_act[cur], 0, lane);
+    lds_b8* const out_l = lds_lane(s_act[cur ^ 1], 2 * nb0 * 64, lane);  // unused for layer 1
+    if constexpr (s == 0) {
+      // accumulators start from the sigma-head term W_sigma dz_sigma
+      const uint32_t hs_off = 4u * (uint32_t)(g * C::KH);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        floatx16 hsv;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const floatx4 w = ldf4(rs, hs_off, P.p_head_sigma + 4 * (16 * (nb0 + i) + 4 * a));
+#pragma unroll
+          for (int r = 0; r < 4; ++r) hsv[4 * a + r] = w[r];
+        }
+#pragma unroll
+        for (int sb = 0; sb < kBlocks; ++sb) {
+          const float dzs = s_dzs[sb * 32 + j];
+#pragma unroll
+          for (int e = 0; e < 16; ++e) acc[sb][i][e] = hsv[e] * dzs;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int sb = 0; sb < kBlocks; ++sb)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) acc[sb][i][e] = 0.f;
+    }
+    constexpr int H = kT / 2;
+    run_steps<0, H, false>(acc, wlo, cb_unused, in_l, in_l, none);
+    if constexpr (layer > 1) load_w<H, 0>(wlo, rs, P.p_bwd[layer - 1], C::NB, nb0, lane_off);
+    run_steps<H, H, false>(acc, whi, cb_unused, in_l, in_l,
+                           [&](int sb) {
+                             epilogue(std::true_type{}, std::integral_constant<bool, (layer > 1)>{}, out_l,
+                                      P.st_h[layer - 1], P.dz_h[layer - 1], sb);
+                           });
+    if constexpr (layer > 1) {
+      load_w<H, H>(whi, rs, P.p_bwd[layer - 1], C::NB, nb0, lane_off);
+      __syncthreads();
+    }
+  });
+}
+
+template <int W, int D, int SKIP, int LX, int LD>
+int launch_bwd_bf16ws(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,
+                      const float* d_sigma, const float* rgb, const float* sigma, const void* stash,
+                      void* dz, hipStream_t s) {
+  const int64_t nwg = (M + 127) / 128;
+  mlp_bwd_bf16ws_kernel<W, D, SKIP, LX, LD><<<(unsigned)nwg, 256, 0, s>>>(
+      P, reinterpret_cast<const char*>(packed), M, d_rgb, d_sigma, rgb, sigma,
+      reinterpret_cast<const __bf16*>(stash), reinterpret_cast<__bf16*>(dz));
+  return 0;
+}

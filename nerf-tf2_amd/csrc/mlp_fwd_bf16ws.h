@@ -38,7 +38,6 @@ constexpr int kSlab = kT * 64;   // bf16x8 per sample block in one activation bu
 constexpr int kHalf = 10;        // register k-steps per half (20-step skip layer)
 
 typedef short shortx8 __attribute__((ext_vector_type(8)));
-
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // The packed weights as a buffer resource: every weight / bias load is a
@@ -79,6 +78,19 @@ __device__ __forceinline__ floatx16 bias_acc(__amdgpu_buffer_rsrc_t rs, int64_t 
     for (int r = 0; r < 4; ++r) c[4 * a + r] = v[r];
   }
   return c;
+}
+
+// Between the accumulator (= B-operand) layout and row chunks.  Of a 16-feature
+// group, lane (j, g) of the accumulator layout holds features {4g .. 4g+3,
+// 8+4g .. 8+4g+3} of sample j (dwords d0 .. d3); a row chunk is the 8 consecutive
+// features 8g .. 8g+7, one 16-byte store or load.  Two v_permlane32_swap trade
+// the middle dwords between lanes j and j + 32; the map is its own inverse.
+__device__ __forceinline__ bf16x8 swap_halves(bf16x8 v) {
+  const u32x4 d = __builtin_bit_cast(u32x4, v);
+  const auto a = __builtin_amdgcn_permlane32_swap(d[0], d[2], false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(d[1], d[3], false, false);
+  const u32x4 r = {a[0], b[0], a[1], b[1]};
+  return __builtin_bit_cast(bf16x8, r);
 }
 
 __device__ __forceinline__ bf16x8 relu_bf16(bf16x8 v) {
@@ -223,12 +235,10 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
       out[sb * kSlab + (2 * i) * 64] = lo;
       out[sb * kSlab + (2 * i + 1) * 64] = hi;
       if (STASH) {
-        // acc element 4a + r <-> feature 32 (nb0 + i) + 8 a + 4 g + r; lo holds a = 0, 1, hi a = 2, 3
-        __bf16* rr = row(sb) + st_off + 32 * (nb0 + i) + 4 * g;
-        *reinterpret_cast<bf16x4*>(rr + 0) = __builtin_shufflevector(lo, lo, 0, 1, 2, 3);
-        *reinterpret_cast<bf16x4*>(rr + 8) = __builtin_shufflevector(lo, lo, 4, 5, 6, 7);
-        *reinterpret_cast<bf16x4*>(rr + 16) = __builtin_shufflevector(hi, hi, 0, 1, 2, 3);
-        *reinterpret_cast<bf16x4*>(rr + 24) = __builtin_shufflevector(hi, hi, 4, 5, 6, 7);
+        // as row chunks: features 32 (nb0 + i) + 16 h + 8 g .. + 7 (lo: h = 0, hi: h = 1)
+        __bf16* rr = row(sb) + st_off + 32 * (nb0 + i) + 8 * g;
+        *reinterpret_cast<bf16x8*>(rr) = swap_halves(lo);
+        *reinterpret_cast<bf16x8*>(rr + 16) = swap_halves(hi);
       }
     }
   };
@@ -311,12 +321,9 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) a[r] = (float)(__bf16)(a[r] > 0.f ? a[r] : 0.f);
       if (STASH) {
-        __bf16* rr = row(sb) + P.st_rbr + 32 * wv + 4 * g;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const floatx4_ v = {a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]};
-          *reinterpret_cast<bf16x4*>(rr + 8 * q) = __builtin_convertvector(v, bf16x4);
-        }
+        __bf16* rr = row(sb) + P.st_rbr + 32 * wv + 8 * g;
+        *reinterpret_cast<bf16x8*>(rr) = swap_halves(acc_to_bf16(a, 0));
+        *reinterpret_cast<bf16x8*>(rr + 16) = swap_halves(acc_to_bf16(a, 1));
       }
       float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll

@@ -191,8 +191,9 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
     P.st_rbr = s; s += 32 * P.NBR;
     P.st_encd = s; s += 2 * P.KD;
   }
-  P.stash_per_sample = s;
-  // dz (floats per sample)
+  P.stash_per_sample = ru((int)s, 8);
+  // dz (elements per sample: float32 for the fp32 form, bf16 for the bf16 form);
+  // every segment starts on a multiple of 8 elements (16-byte aligned bf16 loads)
   int64_t z = 0;
   for (int i = 0; i < D; ++i) { P.dz_h[i] = z; z += W; }
   P.dz_bott = P.dz_rbr = P.dz_rgb = -1;
@@ -200,9 +201,8 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
     P.dz_bott = z; z += W;
     P.dz_rbr = z; z += 32 * P.NBR;
   }
-  P.dz_sigma = z; z += 1;
-  if (P.rgb) { P.dz_rgb = z; z += 3; }
-  z = ru((int)z, 4);
+  P.dz_sigma = z; z += 8;
+  if (P.rgb) { P.dz_rgb = z; z += 8; }
   P.dz_per_sample = z;
   return 0;
 }

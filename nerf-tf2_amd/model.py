@@ -209,7 +209,8 @@ class NerfMLP:
 
     # -- backward ----------------------------------------------------------
     def _backward_into(self, M, g_rgb, g_sigma, rgb, sigma, stash, grad, accumulate, work=None):
-        dz = torch.empty((M, self.dz_per_sample), dtype=torch.float32, device=self.device)
+        dz = torch.empty((M, self.dz_per_sample), dtype=torch.float32 if self.precision == "fp32" else torch.bfloat16,
+                         device=self.device)
         c = lambda t: None if t is None else t.contiguous()
         if g_sigma is None:
             g_sigma = torch.zeros((M, 1), dtype=torch.float32, device=self.device)

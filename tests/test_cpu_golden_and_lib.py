@@ -47,6 +47,15 @@ def test_size_queries(nerf):
     assert lib.nerf_mlp_param_count(ctypes.byref(bad)) == -1
     prop = nerf.MLPConfig(depth=4, width=64, skip=-1, l_xyz=8, l_dir=0).desc()
     assert lib.nerf_mlp_param_count(ctypes.byref(prop)) == 51 * 64 + 64 + 3 * (64 * 64 + 64) + 64 + 1
+    # weight-gradient workspace: the bf16 GEMM runs up to 8 same-shape layers in one
+    # launch, each with at least one 256 x 256 partial tile, even for tiny M
+    tile = (256 * 256 + 256) * 4
+    for M in (1, 100, 4099, 2048 * 5):
+        assert lib.nerf_mlp_wgrad_workspace_bytes(ctypes.byref(d), M) >= 8 * tile
+    assert lib.nerf_mlp_wgrad_workspace_bytes(ctypes.byref(d), 10 ** 7) >= 256 * tile
+    # dz and stash rows start every segment on 8 elements (16-byte bf16 row chunks)
+    assert lib.nerf_mlp_dz_floats_per_sample(ctypes.byref(d)) % 8 == 0
+    assert lib.nerf_mlp_stash_floats_per_sample(ctypes.byref(d)) % 8 == 0
 
 
 def test_invalid_arguments_rejected_without_launch(nerf):

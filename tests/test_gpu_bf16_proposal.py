@@ -19,51 +19,57 @@ def _r16(x):
     return x.bfloat16().float()
 
 
-class _Bf16Linear(torch.autograd.Function):
-    """The bf16 kernels' matmul: y = r(x) @ r(W) accumulated in fp32; the
-    backward dX = r(dy) @ r(W)^T (dz is the bf16 B operand) and dW = r(x)^T @ r(dy)
-    (the bf16 weight GEMM reads the bf16 stash and rounds the fp32 dz rows);
-    bias gradients sum the fp32 dz."""
+class _Bf16Affine(torch.autograd.Function):
+    """The bf16 kernels' dense layer: y = r(x) @ r(W) + b accumulated in fp32.
+    Backward: dz = r(dy) (the chain kernel writes bf16 dz rows and feeds the same
+    rounding to the next MFMA), dX = dz @ r(W)^T, dW = r(x)^T dz, db = sum dz."""
 
     @staticmethod
-    def forward(ctx, x, w):
+    def forward(ctx, x, w, b):
         xr, wr = _r16(x), _r16(w)
         ctx.save_for_backward(xr, wr)
-        return xr @ wr
+        return xr @ wr + b
 
     @staticmethod
     def backward(ctx, dy):
         xr, wr = ctx.saved_tensors
-        return _r16(dy) @ wr.T, xr.T @ _r16(dy)
+        dz = _r16(dy)
+        return dz @ wr.T, xr.T @ dz, dz.sum(0)
 
 
-class _RoundSTE(torch.autograd.Function):
-    """bf16 rounding of the activations the fp32 heads read; identity backward."""
+class _Bf16Head(torch.autograd.Function):
+    """The fp32 sigma / rgb heads on bf16-rounded activations: y = r(x) @ W + b.
+    Backward: dX = dy @ W^T in fp32 (the chain kernel), dW = r(x)^T r(dy) and
+    db = sum r(dy) (the weight GEMM reads the bf16 dz rows)."""
 
     @staticmethod
-    def forward(ctx, x):
-        return _r16(x)
+    def forward(ctx, x, w, b):
+        xr = _r16(x)
+        ctx.save_for_backward(xr, w)
+        return xr @ w + b
 
     @staticmethod
-    def backward(ctx, g):
-        return g
+    def backward(ctx, dy):
+        xr, w = ctx.saved_tensors
+        dz = _r16(dy)
+        return dy @ w.T, xr.T @ dz, dz.sum(0)
 
 
 def _bf16_emulated_mlp(oracle, W, xyz, dirs):
     """model.py:355-392 with the bf16 kernels' rounding points (fp32 biases,
-    activation functions and sigma / rgb heads)."""
+    activation functions and sigma / rgb heads; bf16 dz)."""
     ex = oracle.positional_encoding(xyz, 10)
     ed = oracle.positional_encoding(dirs, 4)
-    lin = _Bf16Linear.apply
+    lin, head = _Bf16Affine.apply, _Bf16Head.apply
     h = ex
     for i in range(8):
-        h = torch.relu(lin(h, W[2 * i]) + W[2 * i + 1])
+        h = torch.relu(lin(h, W[2 * i], W[2 * i + 1]))
         if i == 4:
             h = torch.cat([h, ex], -1)
-    sigma = torch.relu(_RoundSTE.apply(h) @ W[16] + W[17])
-    bott = lin(h, W[18]) + W[19]
-    h9 = torch.relu(lin(torch.cat([bott, ed], -1), W[20]) + W[21])
-    rgb = torch.sigmoid(_RoundSTE.apply(h9) @ W[22] + W[23])
+    sigma = torch.relu(head(h, W[16], W[17]))
+    bott = lin(h, W[18], W[19])
+    h9 = torch.relu(lin(torch.cat([bott, ed], -1), W[20], W[21]))
+    rgb = torch.sigmoid(head(h9, W[22], W[23]))
     return rgb, sigma
 
 
