@@ -32,6 +32,13 @@ MLP_FLOP_PER_SAMPLE = 2 * 593408          # both dense stacks of one model, per 
 RENDER_FLOP_PER_RAY = (64 + 192) * MLP_FLOP_PER_SAMPLE
 
 
+# HBM bytes per fine-model MLP launch (FETCH_SIZE + WRITE_SIZE, each scaled by the
+# same counter on a 1 GiB device copy), from the rocprofv3 --pmc runs whose commands
+# and figures are in profiles/mlp_fwd_f32_render800.md and
+# profiles/mlp_fwd_bf16ws_render800.md.
+TRAFFIC_BYTES_PER_LAUNCH = {"fp32": 454.6e6, "bf16": 162.6e6}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,10 +137,11 @@ def main():
         "train_rays_per_s": head["train_rays_per_s"],
         "train_ms_per_step": head["train_ms_per_step"],
         "render_tflops": round(head["render_rays_per_s"] * RENDER_FLOP_PER_RAY / 1e12, 2),
-        "roofline": {"bound": "mfma", "kernel": "mlp_fwd (fine model, 32,768 rays x 192 samples per launch)",
-                     "achieved": head["mlp_tflops"], "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(head["mlp_tflops"] / peak, 4), "traffic": None,
-                     "avg_launch_ms": head["mlp_avg_launch_ms"], "flop_per_launch": head["mlp_flop_per_launch"]},
+        "roofline": None if "mlp_tflops" not in head else {
+            "bound": "mfma", "kernel": "mlp_fwd (fine model, 32,768 rays x 192 samples per launch)",
+            "achieved": head["mlp_tflops"], "peak": peak, "unit": "TFLOP/s",
+            "frac": round(head["mlp_tflops"] / peak, 4), "traffic": TRAFFIC_BYTES_PER_LAUNCH[prec],
+            "avg_launch_ms": head["mlp_avg_launch_ms"], "flop_per_launch": head["mlp_flop_per_launch"]},
         "cpu_baseline": cpu,
         "forms": results,
     }

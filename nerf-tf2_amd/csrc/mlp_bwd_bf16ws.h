@@ -12,10 +12,10 @@
 // Each dz is written as a bf16 row (the weight-gradient GEMMs read them) and,
 // rounded the same way, is the next MFMA's B operand.  The sigma-head term is
 // the accumulators' starting value (an outer product of the head weights and
-// dz_sigma).  ReLU masks read the bf16 activations of the training stash; the
-// mask is applied to the packed bf16 result with integer max / min / and.  Rows
-// are read and written as 16-byte chunks of 8 consecutive features, swapped
-// into / out of the accumulator layout across lane halves (swap_halves).
+// dz_sigma).  ReLU masks are the bit masks the forward wrote into the stash
+// (one dword per wave, lane and layer), applied to the packed bf16 result.  dz
+// rows are written as 16-byte chunks of 8 consecutive features, swapped out of
+// the accumulator layout across lane halves (swap_halves).
 // Stash reads and dz writes go through buffer resources that end at the last
 // sample, so lanes past it read zeros and their stores are dropped.
 #pragma once
@@ -26,21 +26,22 @@ namespace wsb {
 using namespace ws;
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-// x where h > 0 (both bf16, compared as signed 16-bit integers), else +0.
-__device__ __forceinline__ bf16x8 mask_bf16(bf16x8 x, bf16x8 h) {
-  const shortx8 z = {0, 0, 0, 0, 0, 0, 0, 0}, one = {1, 1, 1, 1, 1, 1, 1, 1};
-  const shortx8 t = __builtin_elementwise_min(__builtin_elementwise_max(__builtin_bit_cast(shortx8, h), z), one);
-  return __builtin_bit_cast(bf16x8, __builtin_bit_cast(shortx8, x) & (z - t));
-}
-
 __device__ __forceinline__ void st4(bf16x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, (int)soff, 0);
 }
 
-// A 16-feature group of a row in the accumulator layout: one 16-byte row-chunk
-// load per lane (features 8 g .. 8 g + 7 at byte offset voff + soff), swapped.
-__device__ __forceinline__ bf16x8 ld_group(__amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
-  return swap_halves(__builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, voff, (int)soff, 0)));
+// v with the ReLU mask applied: the forward's bit mask w holds, for the pairs of
+// accumulator elements kk = 4 h + k (dword k of the half h), bits shift + kk and
+// 16 + shift + kk.
+__device__ __forceinline__ bf16x8 mask_bits(bf16x8 v, uint32_t w, int shift) {
+  u32x4 d = __builtin_bit_cast(u32x4, v);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d[k] &= ((w >> (shift + k)) & 0x00010001u) * 0xFFFFu;
+  return __builtin_bit_cast(bf16x8, d);
+}
+
+__device__ __forceinline__ uint32_t ld_u32(__amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, voff, (int)soff, 0);
 }
 
 // Stores a 16-feature group given in the accumulator layout as row chunks.
@@ -74,10 +75,11 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
       const_cast<__bf16*>(stash + base * sps), 0, (int)(nrows * sps * 2), 0x00020000);
   const __amdgpu_buffer_rsrc_t rdz =
       __builtin_amdgcn_make_buffer_rsrc(dz + base * zps, 0, (int)(nrows * zps * 2), 0x00020000);
-  // per-lane byte offsets of this lane's sample row in sample block sb (+ 8 g
-  // features: the lane's row chunk of each 16-feature group)
-  auto st_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps + 8 * g) * 2); };
+  // per-lane byte offsets in the rows of sample block sb: the lane's dz row chunk
+  // of each 16-feature group (+ 8 g features), its stash mask word, the row start
   auto dz_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * zps + 8 * g) * 2); };
+  auto mask_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps + 2 * g) * 2); };  // this lane's mask word
+  auto row0 = [&](int sb) { return (uint32_t)((sb * 32 + j) * sps * 2); };
 
   bf16x8 wlo[kHalf][2], whi[kHalf][2];
   const floatx16 cb_unused[2] = {};
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
       dzr[c] = dy * y * (1.f - y);
     }
     if (g == 0) s_dzs[wv * 32 + j] = dzs;
-    const uint32_t zr = dz_row(wv), sr = st_row(wv);
+    const uint32_t zr = dz_row(wv), r0 = row0(wv);
     if (g == 0) {
       const bf16x4 hz = {(__bf16)dzs, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f};
       const bf16x4 rz = {(__bf16)dzr[0], (__bf16)dzr[1], (__bf16)dzr[2], (__bf16)0.f};
@@ -121,9 +123,13 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
         const floatx4 w = hr4[16 * nb + i];
         a[i] = dzr[0] * w[0] + dzr[1] * w[1] + dzr[2] * w[2];
       }
+      // the forward's 16 ReLU bits of rgb-branch block nb (lane halves in one dword)
+      const uint32_t hw = ld_u32(rst, r0, (P.st_mask + 16 * D + 2 * nb) * 2);
+      const uint32_t u = g ? hw >> 16 : hw & 0xFFFFu;
+      const uint32_t bits = (u & 0xFFu) | ((u & 0xFF00u) << 8);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const bf16x8 v = mask_bf16(acc_to_bf16(a, h), ld_group(rst, sr, (P.st_rbr + 32 * nb + 16 * h) * 2));
+        const bf16x8 v = mask_bits(acc_to_bf16(a, h), bits, 4 * h);
         out[(2 * nb + h) * 64] = v;
         st_group(v, rdz, zr, (P.dz_rbr + 32 * nb + 16 * h) * 2);
       }
@@ -134,16 +140,17 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   // Epilogue of sample block sb: ReLU mask (MASKED) from the stash activations at
   // feature offset st_off, bf16 into LDS (B layout; out may be null) and the dz row
   // at dz_off.
-  auto epilogue = [&](auto masked_c, auto lds_c, lds_b8* out, int64_t st_off, int64_t dz_off, int sb) {
+  auto epilogue = [&](auto masked_c, auto lds_c, lds_b8* out, int64_t mask_off, int64_t dz_off, int sb) {
     constexpr bool MASKED = decltype(masked_c)::value, TO_LDS = decltype(lds_c)::value;
-    const uint32_t zr = dz_row(sb), sr = st_row(sb);
+    const uint32_t zr = dz_row(sb);
+    const uint32_t bits = MASKED ? ld_u32(rst, mask_row(sb), (mask_off + 4 * wv) * 2) : 0u;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int64_t f0 = 32 * (nb0 + i);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         bf16x8 v = acc_to_bf16(acc[sb][i], h);
-        if constexpr (MASKED) v = mask_bf16(v, ld_group(rst, sr, (st_off + f0 + 16 * h) * 2));
+        if constexpr (MASKED) v = mask_bits(v, bits, 8 * i + 4 * h);
         if constexpr (TO_LDS) out[sb * kSlab + (2 * i + h) * 64] = v;
         st_group(v, rdz, zr, (dz_off + f0 + 16 * h) * 2);
       }
@@ -209,7 +216,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
     run_steps<H, H, false>(acc, whi, cb_unused, in_l, in_l,
                            [&](int sb) {
                              epilogue(std::true_type{}, std::integral_constant<bool, (layer > 1)>{}, out_l,
-                                      P.st_h[layer - 1], P.dz_h[layer - 1], sb);
+                                      P.st_mask + 16 * (layer - 1), P.dz_h[layer - 1], sb);
                            });
     if constexpr (layer > 1) {
       load_w<H, H>(whi, rs, P.p_bwd[layer - 1], C::NB, nb0, lane_off);

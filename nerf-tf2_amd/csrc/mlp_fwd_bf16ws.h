@@ -93,6 +93,26 @@ __device__ __forceinline__ bf16x8 swap_halves(bf16x8 v) {
   return __builtin_bit_cast(bf16x8, r);
 }
 
+typedef unsigned short ushortx2 __attribute__((ext_vector_type(2)));
+
+// Nonzero bits of a block's 16 accumulator elements (lo: elements 0..7, hi:
+// 8..15; non-negative bf16 after ReLU, so nonzero == positive): the pair
+// (2k, 2k+1) goes to bits k and 16 + k, k = 0..7.
+__device__ __forceinline__ uint32_t nz_bits(bf16x8 lo, bf16x8 hi) {
+  const u32x4 a = __builtin_bit_cast(u32x4, lo), b = __builtin_bit_cast(u32x4, hi);
+  const ushortx2 one = {1, 1};
+  uint32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    // (the elements are copied out first: a bit_cast of a vector element
+    // subscript reads element 0 with this compiler)
+    const uint32_t ak = a[k], bk = b[k];
+    w |= __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(ushortx2, ak), one)) << k;
+    w |= __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(ushortx2, bk), one)) << (4 + k);
+  }
+  return w;
+}
+
 __device__ __forceinline__ bf16x8 relu_bf16(bf16x8 v) {
   const shortx8 z = {0, 0, 0, 0, 0, 0, 0, 0};
   return __builtin_bit_cast(bf16x8, __builtin_elementwise_max(__builtin_bit_cast(shortx8, v), z));
@@ -224,7 +244,11 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
 
   // Epilogue of sample block sb of a 256-wide layer: bf16, ReLU, LDS (B layout), stash.
   // out: per-lane pointer at k-step 2 nb0 of sample block 0 of the output buffer
-  auto epilogue = [&](lds_b8* out, bool relu, int64_t st_off, int sb) {
+  // MASK: also the ReLU bit mask of the wave's 64 features (bits of output
+  // block nb0 + i at 8 i + {0..7, 16..23}) at stash offset mask_off
+  auto epilogue = [&](lds_b8* out, bool relu, int64_t st_off, auto mask_c, int64_t mask_off, int sb) {
+    constexpr bool MASK = decltype(mask_c)::value;
+    uint32_t bits = 0;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       bf16x8 lo = acc_to_bf16(acc[sb][i], 0), hi = acc_to_bf16(acc[sb][i], 1);
@@ -239,8 +263,10 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
         __bf16* rr = row(sb) + st_off + 32 * (nb0 + i) + 8 * g;
         *reinterpret_cast<bf16x8*>(rr) = swap_halves(lo);
         *reinterpret_cast<bf16x8*>(rr + 16) = swap_halves(hi);
+        bits |= nz_bits(lo, hi) << (8 * i);
       }
     }
+    if constexpr (STASH && MASK) *reinterpret_cast<uint32_t*>(row(sb) + mask_off + 4 * wv + 2 * g) = bits;
   };
 
   __syncthreads();  // encodings visible
@@ -248,7 +274,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
   lds_b8* const ex_l = lds_lane(s_ex, 0, lane);
   {
     lds_b8* const out_l = lds_lane(s_act[0], 2 * nb0 * 64, lane);
-    run_steps<kT, kTX, true>(acc, whi, cb, nullptr, ex_l, [&](int sb) { epilogue(out_l, true, P.st_h[0], sb); });
+    run_steps<kT, kTX, true>(acc, whi, cb, nullptr, ex_l, [&](int sb) { epilogue(out_l, true, P.st_h[0], std::true_type{}, P.st_mask, sb); });
   }
   load_half(whi, 1, SKIP == 0, true);  // layer 1's second half and bias
   cb[0] = bias_acc(rs, P.p_bias[1], nb0, g_off);
@@ -298,7 +324,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
       cb[0] = bias_acc(rs, P.p_bias[D + 1], wv, g_off);
     }
     const int64_t st_off = l < D ? P.st_h[l] : P.st_bott;
-    run_steps<H, H, false>(acc, whi, cb, in_l, ex_l, [&](int sb) { epilogue(out_l, l < D, st_off, sb); });
+    run_steps<H, H, false>(acc, whi, cb, in_l, ex_l, [&](int sb) { epilogue(out_l, l < D, st_off, std::integral_constant<bool, (l < D)>{}, P.st_mask + 16 * l, sb); });
     if constexpr (l < D) load_half(whi, l + 1, l + 1 == SKIP + 1, true);  // whi is free
     __syncthreads();  // layer l outputs complete
   });
@@ -322,8 +348,13 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
       for (int r = 0; r < 16; ++r) a[r] = (float)(__bf16)(a[r] > 0.f ? a[r] : 0.f);
       if (STASH) {
         __bf16* rr = row(sb) + P.st_rbr + 32 * wv + 8 * g;
-        *reinterpret_cast<bf16x8*>(rr) = swap_halves(acc_to_bf16(a, 0));
-        *reinterpret_cast<bf16x8*>(rr + 16) = swap_halves(acc_to_bf16(a, 1));
+        const bf16x8 lo = acc_to_bf16(a, 0), hi = acc_to_bf16(a, 1);
+        *reinterpret_cast<bf16x8*>(rr) = swap_halves(lo);
+        *reinterpret_cast<bf16x8*>(rr + 16) = swap_halves(hi);
+        // ReLU bits of rgb-branch block wv: bits 0..7 and 16..23 of nz_bits as 16 bits
+        const uint32_t b = nz_bits(lo, hi);
+        *reinterpret_cast<unsigned short*>(row(sb) + P.st_mask + 16 * D + 2 * wv + g) =
+            (unsigned short)((b & 0xFFu) | ((b >> 8) & 0xFF00u));
       }
       float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll

@@ -69,6 +69,11 @@ struct MlpPlan {
   int prec;
   // stash (elements per sample; fp32 or bf16 elements) and dz (floats per sample)
   int64_t st_encx, st_h[NERF_MAX_LAYERS], st_bott, st_rbr, st_encd, stash_per_sample;
+  // ReLU bit masks of the weight-stationary bf16 kernels (W = 256 with an rgb
+  // branch; -1 otherwise): 16 elements (32 bytes, one uint32 per wave and lane
+  // half) per trunk layer, then 8 elements for the rgb branch (one uint16 per
+  // 32-feature block and lane half)
+  int64_t st_mask;
   int64_t dz_h[NERF_MAX_LAYERS], dz_bott, dz_rbr, dz_sigma, dz_rgb, dz_per_sample;
 };
 
@@ -190,6 +195,12 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
     P.st_bott = s; s += W;
     P.st_rbr = s; s += 32 * P.NBR;
     P.st_encd = s; s += 2 * P.KD;
+  }
+  s = ru((int)s, 8);
+  P.st_mask = -1;
+  if (W == 256 && P.rgb && P.NBR == 4) {
+    P.st_mask = s;
+    s += 16 * D + 8;
   }
   P.stash_per_sample = ru((int)s, 8);
   // dz (elements per sample: float32 for the fp32 form, bf16 for the bf16 form);

@@ -48,8 +48,8 @@ class _P:
 
 class ProposalNeRF:
     def __init__(self, params, precision: str = "bf16", seed: int = 0, device="cuda", n_prop: int = 48,
-                 fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 1.0,
-                 dilate: int = 2):
+                 fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 10.0,
+                 dilate: int = 2, proposal_config: MLPConfig = PROPOSAL_CONFIG):
         self.params = params
         self.white_bg = bool(params.system.white_bg)
         self.precision = precision
@@ -61,7 +61,7 @@ class ProposalNeRF:
         self.interlevel_weight = interlevel_weight
         self.dilate = dilate
         self.p_prop = _P(n_prop, self.n_fine, params.sampling.lin_inv_depth, self.white_bg)
-        self.proposal = NerfMLP("proposal", PROPOSAL_CONFIG, precision, seed=seed + 2, device=device)
+        self.proposal = NerfMLP("proposal", proposal_config, precision, seed=seed + 2, device=device)
         self.coarse_model = NerfMLP("coarse", MLPConfig(), precision, seed=seed, device=device)
         self.fine_model = NerfMLP("fine", MLPConfig(), precision, seed=seed + 1, device=device)
         self.optimizer = AdamTF(self.models)
