@@ -393,7 +393,15 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   constexpr bool kModeN = NB >= 4;
   constexpr int RB = kModeN ? KB : (KB >= 8 ? 2 : 1);  // row blocks per wave
   constexpr int CB = kModeN ? NB / 4 : NB;             // column blocks per wave
-  constexpr int RX = (KB + 1) / 2, RZ = (NB + 1) / 2;  // staging rounds (64 features each)
+  // Staging: thread t loads 16 bytes (8 features, feature group fg = t % NFG) of two
+  // consecutive sample rows (sample pair ps + (256 / NFG) i, round i < NFG / 8), so
+  // a wave reads whole 16 B x NFG row segments; the transposed LDS tile stores
+  // sample pair p of feature row R at pair position p ^ 4 ((R >> 4) & 7), which
+  // keeps every 8-sample MFMA operand contiguous and spreads the pair writes of a
+  // wave over the banks.
+  constexpr int NFGX = KB >= 8 ? 32 : KB >= 4 ? 16 : 8, NFGZ = NB >= 8 ? 32 : NB >= 4 ? 16 : 8;
+  constexpr int RX = NFGX / 8, RZ = NFGZ / 8;
+  constexpr int PPX = 256 / NFGX, PPZ = 256 / NFGZ;  // sample pairs per round
   typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kWF][kWP];
   __shared__ __attribute__((aligned(16))) __bf16 Zt[2][kWF][kWP];
@@ -404,7 +412,9 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   const int64_t rows = me > mb ? me - mb : 0;
   const int t = threadIdx.x, lane = t & 63, r = lane & 31, q = lane >> 5;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int pair = t & 31, fg0 = t >> 5;  // staging: sample pair, feature group (+ 8 per round)
+  const int fgx = t % NFGX, psx = t / NFGX, fgz = t % NFGZ, psz = t / NFGZ;
+  const int swx = 4 * ((fgx >> 1) & 7), swz = 4 * ((fgz >> 1) & 7);
+  const int qx = q ^ (r >> 4);  // operand reads: swizzled 8-sample group of this lane
   const __amdgpu_buffer_rsrc_t rx = rows_rsrc(a.X + mb * a.ldx, rows * a.ldx * 2);
   const __amdgpu_buffer_rsrc_t rz = rows_rsrc(a.Z + mb * a.ldz, rows * a.ldz * 2);
 
@@ -415,58 +425,61 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
     for (int c = 0; c < CB; ++c)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][c][e] = 0.f;
-  float bsum[RZ][8];
+  float bsum[8];
 #pragma unroll
-  for (int i = 0; i < RZ; ++i)
-#pragma unroll
-    for (int e = 0; e < 8; ++e) bsum[i][e] = 0.f;
+  for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
 
   u32x4_ xr[RX][2], zr[RZ][2];
   auto load_chunk = [&](int c) {
-    const uint32_t m0 = (uint32_t)(c * kWC + 2 * pair);
 #pragma unroll
     for (int i = 0; i < RX; ++i)
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        xr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
-            rx, (uint32_t)(((m0 + h) * a.ldx + J.xoff + 8 * (fg0 + 8 * i)) * 2), 0, 0);
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t m = (uint32_t)(c * kWC + 2 * (psx + PPX * i) + h);
+        xr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(rx, (uint32_t)((m * a.ldx + J.xoff + 8 * fgx) * 2), 0, 0);
+      }
 #pragma unroll
     for (int i = 0; i < RZ; ++i)
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        zr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
-            rz, (uint32_t)(((m0 + h) * a.ldz + J.zoff + 8 * (fg0 + 8 * i)) * 2), 0, 0);
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t m = (uint32_t)(c * kWC + 2 * (psz + PPZ * i) + h);
+        zr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(rz, (uint32_t)((m * a.ldz + J.zoff + 8 * fgz) * 2), 0, 0);
+      }
   };
   auto store_chunk = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < RX; ++i) {
       const bf16x8 v0 = __builtin_bit_cast(bf16x8, xr[i][0]), v1 = __builtin_bit_cast(bf16x8, xr[i][1]);
+      const int col = 2 * ((psx + PPX * i) ^ swx);
 #pragma unroll
-      for (int e = 0; e < 8; ++e)
-        *reinterpret_cast<bf16x2*>(&Xt[buf][8 * (fg0 + 8 * i) + e][2 * pair]) = bf16x2{v0[e], v1[e]};
+      for (int e = 0; e < 8; ++e) *reinterpret_cast<bf16x2*>(&Xt[buf][8 * fgx + e][col]) = bf16x2{v0[e], v1[e]};
     }
 #pragma unroll
     for (int i = 0; i < RZ; ++i) {
       const bf16x8 v0 = __builtin_bit_cast(bf16x8, zr[i][0]), v1 = __builtin_bit_cast(bf16x8, zr[i][1]);
+      const int col = 2 * ((psz + PPZ * i) ^ swz);
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        *reinterpret_cast<bf16x2*>(&Zt[buf][8 * (fg0 + 8 * i) + e][2 * pair]) = bf16x2{v0[e], v1[e]};
-        bsum[i][e] += (float)v0[e] + (float)v1[e];
+        *reinterpret_cast<bf16x2*>(&Zt[buf][8 * fgz + e][col]) = bf16x2{v0[e], v1[e]};
+        bsum[e] += (float)v0[e] + (float)v1[e];
       }
     }
   };
   auto row_block = [&](int i) { return kModeN ? i : (wv + 4 * i < KB ? wv + 4 * i : KB - 1); };
   auto col_block = [&](int c) { return kModeN ? wv + 4 * c : c; };
+  // 8 samples 16 s + 8 q .. of feature row 32 blk + r, at its swizzled position
+  auto operand = [&](const __bf16 (*tile)[kWP], int blk, int s) {
+    return *reinterpret_cast<const bf16x8*>(&tile[32 * blk + r][16 * (s ^ (blk & 3)) + 8 * qx]);
+  };
   auto compute = [&](int buf) {
 #pragma unroll
     for (int s = 0; s < kWC / 16; ++s) {
       bf16x8 bz[CB];
 #pragma unroll
-      for (int c = 0; c < CB; ++c)
-        bz[c] = *reinterpret_cast<const bf16x8*>(&Zt[buf][32 * col_block(c) + r][16 * s + 8 * q]);
+      for (int c = 0; c < CB; ++c) bz[c] = operand(Zt[buf], col_block(c), s);
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
-        const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&Xt[buf][32 * row_block(i) + r][16 * s + 8 * q]);
+        const bf16x8 ax = operand(Xt[buf], row_block(i), s);
 #pragma unroll
         for (int c = 0; c < CB; ++c) acc[i][c] = mfma_bf16(ax, bz[c], acc[i][c]);
       }
@@ -501,22 +514,27 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
     }
   }
   if (J.bpart) {
-    // reduce over the 32 sample pairs sharing this feature group (one 32-lane half)
+    // the stores above never touched LDS; the loop ended with a barrier, so Xt is free:
+    // lanes with the same feature group are NFGZ apart (within a wave, then across waves)
 #pragma unroll
-    for (int i = 0; i < RZ; ++i)
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+      for (int msk = NFGZ; msk < 64; msk <<= 1) bsum[e] += shfl_xor(bsum[e], msk);
+    }
+    float* red = reinterpret_cast<float*>(&Xt[0][0][0]);  // [4 waves][NFGZ][8]
+    if (lane < NFGZ) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) red[(wv * NFGZ + lane) * 8 + e] = bsum[e];
+    }
+    __syncthreads();
+    if (wv == 0 && lane < NFGZ) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-#pragma unroll
-        for (int msk = 1; msk < 32; msk <<= 1) bsum[i][e] += shfl_xor(bsum[i][e], msk);
+        const float v = (red[(0 * NFGZ + lane) * 8 + e] + red[(1 * NFGZ + lane) * 8 + e]) +
+                        (red[(2 * NFGZ + lane) * 8 + e] + red[(3 * NFGZ + lane) * 8 + e]);
+        const int n = 8 * lane + e;
+        if (n < J.N) J.bpart[(int64_t)split * J.N + n] = v;
       }
-    if (pair == 0) {
-#pragma unroll
-      for (int i = 0; i < RZ; ++i)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int n = 8 * (fg0 + 8 * i) + e;
-          if (n < J.N) J.bpart[(int64_t)split * J.N + n] = bsum[i][e];
-        }
     }
   }
 }

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
       }
     }
   };
-  auto none = [](int) {};
+  const NoEpilogue none;
 
   // ---- rgb branch: d bott = W_{D+1} dz_rbr (linear; 8 k-steps, halves of 4)
   {

@@ -138,22 +138,34 @@ __device__ __forceinline__ bf16x8 b_operand(const lds_b8* in, const lds_b8* ex, 
 // acc[sb][i] (+)= sum over k-steps T0 .. T0+H-1 of w[k - T0][i] * B(sb, k), for
 // all four sample blocks.  INIT: the first k-step starts from the bias cb.
 // After the last k-step of each block, epi(sb) runs (no-op for a first half).
+struct NoEpilogue {
+  __device__ void operator()(int) const {}
+};
+
+// Without an epilogue (a first half) step s runs sample block pair p = s / 2H,
+// k-step (s / 2) % H, block 2p + s % 2: the two blocks of a pair alternate, so
+// consecutive MFMAs go to four independent accumulators.  With an epilogue the
+// blocks run one after the other, so each block's epilogue overlaps the next
+// block's MFMAs.
 template <int T0, int H, bool INIT, typename Epi>
 __device__ __forceinline__ void run_steps(floatx16 (&acc)[kBlocks][2], const bf16x8 (&w)[kHalf][2],
                                           const floatx16 (&cb)[2], const lds_b8* in, const lds_b8* ex,
                                           Epi&& epi) {
   constexpr int S = kBlocks * H;
+  constexpr bool PAIRS = std::is_same_v<std::decay_t<Epi>, NoEpilogue>;
+  auto blk = [](int s) { return PAIRS ? 2 * (s / (2 * H)) + (s & 1) : s / H; };
+  auto kst = [](int s) { return PAIRS ? (s >> 1) % H : s % H; };
   bf16x8 b[4];
 #pragma unroll
-  for (int s = 0; s < 3; ++s) b[s] = b_operand(in, ex, s / H, T0 + s % H);
+  for (int s = 0; s < 3; ++s) b[s] = b_operand(in, ex, blk(s), T0 + kst(s));
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (s + 3 < S) b[(s + 3) & 3] = b_operand(in, ex, (s + 3) / H, T0 + (s + 3) % H);
-    const int sb = s / H, k = s % H;
+    if (s + 3 < S) b[(s + 3) & 3] = b_operand(in, ex, blk(s + 3), T0 + kst(s + 3));
+    const int sb = blk(s), k = kst(s);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
       acc[sb][i] = mfma_bf16(w[k][i], b[s & 3], (INIT && k == 0) ? cb[i] : acc[sb][i]);
-    if (k == H - 1) epi(sb);
+    if (!PAIRS && k == H - 1) epi(sb);
   }
 }
 
@@ -310,7 +322,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
     }
     constexpr bool enc = l == SKIP + 1;
     constexpr int H = enc ? (kT + kTX) / 2 : kT / 2;
-    run_steps<0, H, true>(acc, wlo, cb, in_l, ex_l, [](int) {});
+    run_steps<0, H, true>(acc, wlo, cb, in_l, ex_l, NoEpilogue{});
     // wlo is free: the next layer's first half and bias (after the bottleneck: the
     // rgb branch's 18 k-steps of output block wv and its bias)
     if constexpr (l < D) {

@@ -2,12 +2,13 @@
 
     python -m nerf-tf2_amd.quality --steps 2000          (run from the repo root)
 
-Trains (graph-replayed steps), on identical 4096-ray batches drawn across 100
-seed-placed 800x800 views, (a) the plain FP32 coarse+fine NeRF, (b) the same in BF16 (precision
-form, config 2) and (c) the BF16 learned-proposal form (config 4); then
-renders held-out 800x800 views with each and reports per-element PSNR and max
-|rgb error| against the float64 ground truth, the errors of (b) and (c)
-against (a), and render / train rays/s.  Prints one JSON object.
+Trains (graph-replayed steps), for each seed, on identical 4096-ray batches
+drawn across 100 seed-placed 800x800 views, (a) the plain FP32 coarse+fine NeRF,
+(b) the same in BF16 (precision form) and (c) the BF16 learned-proposal form;
+then renders held-out 800x800 views with each and reports per-element PSNR and
+max |rgb error| against the float64 ground truth, the errors of (b) and (c)
+against (a), render / train rays/s, and the PSNR change against (a) per seed and
+averaged.  Prints one JSON object.
 """
 from __future__ import annotations
 
@@ -59,7 +60,8 @@ def psnr(a, b):
     return -10.0 * math.log10(max(mse, 1e-30))
 
 
-def run(steps=2000, eval_views=2, seed=0, device="cuda", forms_sel=("plain_fp32", "plain_bf16", "learned_bf16")):
+def run(steps=2000, eval_views=4, seeds=(0, 1), device="cuda",
+        forms_sel=("plain_fp32", "plain_bf16", "learned_bf16")):
     nerf = _pkg()
     sc = nerf.scene
     scene = sc.BlobScene.from_seed(0)
@@ -68,7 +70,6 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda", forms_sel=("plain_fp32"
     batches = make_batches(nerf, scene, steps, device=device)
     near = torch.full((4096, 1), sc.NEAR, device=device)
     far = torch.full((4096, 1), sc.FAR, device=device)
-    t_data = time.perf_counter() - t0
     # held-out views
     H = W = 800
     K = sc.intrinsics(H, W, 1111.1)
@@ -77,75 +78,88 @@ def run(steps=2000, eval_views=2, seed=0, device="cuda", forms_sel=("plain_fp32"
         ro, rd = nerf.ray_utils.get_rays(H, W, K, pose, device=device)
         gt, _, _ = scene.render(ro.double(), rd.double(), n_samples=1024, chunk=4096)
         views.append((ro, rd, gt))
-    forms = {
-        "plain_fp32": lambda: nerf.NeRF(P, "fp32", seed=seed, device=device),
-        "plain_bf16": lambda: nerf.NeRF(P, "bf16", seed=seed, device=device),
-        "learned_bf16": lambda: nerf.ProposalNeRF(P, "bf16", seed=seed, device=device),
+    t_data = time.perf_counter() - t0
+    makers = {
+        "plain_fp32": lambda sd: nerf.NeRF(P, "fp32", seed=sd, device=device),
+        "plain_bf16": lambda sd: nerf.NeRF(P, "bf16", seed=sd, device=device),
+        "learned_bf16": lambda sd: nerf.ProposalNeRF(P, "bf16", seed=sd, device=device),
     }
-    forms = {k: v for k, v in forms.items() if k in forms_sel}
-    res = {"steps": steps, "batch": 4096, "data_seconds": round(t_data, 1)}
+    res = {"steps": steps, "batch": 4096, "eval_views": eval_views, "seeds": list(seeds),
+           "data_seconds": round(t_data, 1), "runs": []}
     renders = {}
-    for name, make in forms.items():
-        net = make()
-        step = nerf.GraphedTrainStep(net, 4096)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(steps):
-            o, d, gt = batches[i]
-            st = step(o, d, near, far, gt)
-            if i % 250 == 0 or i == steps - 1:
-                print(f"quality: {name} step {i} fine_loss {float(st['fine_loss']):.5f}", file=sys.stderr, flush=True)
-        torch.cuda.synchronize()
-        t_train = time.perf_counter() - t0
-        preds, fb = [], []
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for ro, rd, _ in views:
-            n = ro.shape[0]
-            nr = torch.full((n, 1), sc.NEAR, device=device)
-            fr = torch.full((n, 1), sc.FAR, device=device)
-            preds.append(net.predict(ro, rd, nr, fr)["pred_rgb"])
-            fb.append(getattr(net, "last_fallback_fraction", 0.0))
-        torch.cuda.synchronize()
-        t_render = time.perf_counter() - t0
-        renders[name] = preds
-        print(f"quality: {name} rendered {len(preds)} views", file=sys.stderr, flush=True)
-        gts = [v[2] for v in views]
-        res[name] = {
-            "train_rays_per_s": round(4096 * steps / t_train, 1),
-            "render_rays_per_s": round(sum(v[0].shape[0] for v in views) / t_render, 1),
-            "psnr_vs_fp64_gt": [round(psnr(p, g), 3) for p, g in zip(preds, gts)],
-            "max_abs_rgb_err_vs_fp64_gt": [round(float((p.double() - g).abs().max()), 4) for p, g in zip(preds, gts)],
-        }
-        if fb and name == "learned_bf16":
-            res[name]["fallback_fraction"] = [round(x, 4) for x in fb]
-        if getattr(net, "scaler", None) is not None:
-            res[name]["loss_scale"] = net.scaler.scale
-            res[name]["skipped_steps"] = net.scaler.skipped
-        del step, net
-        torch.cuda.empty_cache()
-    for name in ("plain_bf16", "learned_bf16"):
-        if name not in res or "plain_fp32" not in res:
+    for sd in seeds:
+        for name in forms_sel:
+            torch.manual_seed(1000 + sd)
+            net = makers[name](sd)
+            step = nerf.GraphedTrainStep(net, 4096)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(steps):
+                o, d, gt = batches[i]
+                st = step(o, d, near, far, gt)
+                if i % 500 == 0 or i == steps - 1:
+                    print(f"quality: seed {sd} {name} step {i} fine_loss {float(st['fine_loss']):.5f}",
+                          file=sys.stderr, flush=True)
+            torch.cuda.synchronize()
+            t_train = time.perf_counter() - t0
+            preds, fb = [], []
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for ro, rd, _ in views:
+                n = ro.shape[0]
+                nr = torch.full((n, 1), sc.NEAR, device=device)
+                fr = torch.full((n, 1), sc.FAR, device=device)
+                preds.append(net.predict(ro, rd, nr, fr)["pred_rgb"])
+                fb.append(getattr(net, "last_fallback_fraction", 0.0))
+            torch.cuda.synchronize()
+            t_render = time.perf_counter() - t0
+            renders[(sd, name)] = preds
+            rec = {"seed": sd, "form": name,
+                   "train_rays_per_s": round(4096 * steps / t_train, 1),
+                   "render_rays_per_s": round(sum(v[0].shape[0] for v in views) / t_render, 1),
+                   "psnr_vs_fp64_gt": [round(psnr(p, v[2]), 3) for p, v in zip(preds, views)],
+                   "max_abs_rgb_err_vs_fp64_gt": [round(float((p.double() - v[2]).abs().max()), 4)
+                                                  for p, v in zip(preds, views)]}
+            rec["psnr_mean"] = round(float(np.mean(rec["psnr_vs_fp64_gt"])), 3)
+            if name == "learned_bf16":
+                rec["fallback_fraction"] = [round(x, 4) for x in fb]
+            if getattr(net, "scaler", None) is not None:
+                rec["loss_scale"] = net.scaler.scale
+                rec["skipped_steps"] = net.scaler.skipped
+            if name != "plain_fp32" and (sd, "plain_fp32") in renders:
+                ref = renders[(sd, "plain_fp32")]
+                rec["psnr_vs_plain_fp32"] = [round(psnr(a, b), 3) for a, b in zip(preds, ref)]
+                rec["max_abs_rgb_err_vs_plain_fp32"] = [round(float((a - b).abs().max()), 4)
+                                                        for a, b in zip(preds, ref)]
+            res["runs"].append(rec)
+            print(f"quality: {json.dumps(rec)}", file=sys.stderr, flush=True)
+            del step, net
+            torch.cuda.empty_cache()
+    # PSNR change against plain FP32, per seed and averaged
+    summary = {}
+    by = {(r["seed"], r["form"]): r for r in res["runs"]}
+    for name in forms_sel:
+        if name == "plain_fp32" or not all((sd, "plain_fp32") in by for sd in seeds):
             continue
-        res[name]["psnr_vs_plain_fp32"] = [round(psnr(a, b), 3) for a, b in zip(renders[name], renders["plain_fp32"])]
-        res[name]["max_abs_rgb_err_vs_plain_fp32"] = [round(float((a - b).abs().max()), 4)
-                                                      for a, b in zip(renders[name], renders["plain_fp32"])]
-        res[name]["psnr_delta_db"] = round(float(np.mean(res[name]["psnr_vs_fp64_gt"]) -
-                                                 np.mean(res["plain_fp32"]["psnr_vs_fp64_gt"])), 3)
-        res[name]["render_speedup_vs_plain_fp32"] = round(res[name]["render_rays_per_s"] /
-                                                         res["plain_fp32"]["render_rays_per_s"], 2)
-        res[name]["train_speedup_vs_plain_fp32"] = round(res[name]["train_rays_per_s"] /
-                                                        res["plain_fp32"]["train_rays_per_s"], 2)
+        deltas = [round(by[(sd, name)]["psnr_mean"] - by[(sd, "plain_fp32")]["psnr_mean"], 3) for sd in seeds]
+        summary[name] = {"psnr_delta_db_per_seed": deltas, "psnr_delta_db_mean": round(float(np.mean(deltas)), 3),
+                         "render_speedup": round(np.mean([by[(sd, name)]["render_rays_per_s"] /
+                                                          by[(sd, "plain_fp32")]["render_rays_per_s"] for sd in seeds]), 2),
+                         "train_speedup": round(np.mean([by[(sd, name)]["train_rays_per_s"] /
+                                                         by[(sd, "plain_fp32")]["train_rays_per_s"] for sd in seeds]), 2)}
+    res["summary"] = summary
     return res
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--eval-views", type=int, default=2)
+    ap.add_argument("--eval-views", type=int, default=4)
+    ap.add_argument("--seeds", default="0,1")
     ap.add_argument("--forms", default="plain_fp32,plain_bf16,learned_bf16")
     args = ap.parse_args(argv)
-    print(json.dumps(run(args.steps, args.eval_views, forms_sel=tuple(args.forms.split(",")))))
+    print(json.dumps(run(args.steps, args.eval_views, tuple(int(x) for x in args.seeds.split(",")),
+                         forms_sel=tuple(args.forms.split(",")))))
 
 
 if __name__ == "__main__":
