@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--no-graphs", action="store_true",
                     help="run the train steps eagerly instead of replaying a captured HIP graph "
                          "(for kernel traces: the profiler does not follow graph replays)")
+    ap.add_argument("--recompute-rays", type=int, default=16384,
+                    help="train batch of the activation-recomputation measurement (0: skip)")
+    ap.add_argument("--recompute-chunk", type=int, default=2048,
+                    help="rays per recomputation chunk")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -118,6 +122,9 @@ def main():
     for form in forms:
         results[form] = measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches)
     head = results[forms[0]]
+    recompute = None
+    if args.recompute_rays > 0:
+        recompute = {p: measure_recompute(p, args, nerf, P, dev, batches) for p in ("fp32", "bf16")}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -144,6 +151,7 @@ def main():
             "avg_launch_ms": head["mlp_avg_launch_ms"], "flop_per_launch": head["mlp_flop_per_launch"]},
         "cpu_baseline": cpu,
         "forms": results,
+        "recompute": recompute,
     }
     if "learned_bf16" in results and "plain_fp32" in results and results["plain_fp32"]["train_rays_per_s"]:
         line["learned_vs_plain"] = {
@@ -222,6 +230,37 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
         out["mlp_tflops"] = round(flop / (avg_ms * 1e-3) / 1e12, 2)
     del net
     torch.cuda.empty_cache()
+    return out
+
+
+def measure_recompute(prec, args, nerf, P, dev, batches):
+    """Train steps on one large batch (args.recompute_rays rays x 192 samples) with
+    and without activation recomputation: peak HBM allocated by the step and
+    train rays/s (eager steps, 1 untimed + 3 timed)."""
+    import torch
+    n = args.recompute_rays
+    reps = -(-n // batches[0][0].shape[0])
+    big = [torch.cat([batches[i % len(batches)][k] for i in range(reps)])[:n].contiguous() for k in range(5)]
+    out = {"rays": n, "samples_per_ray": 192, "chunk_rays": args.recompute_chunk}
+    for mode in ("stash", "recompute"):
+        net = nerf.NeRF(P, prec, seed=0, device=dev)
+        if mode == "recompute":
+            net.coarse_model.recompute_chunk = args.recompute_chunk
+            net.fine_model.recompute_chunk = args.recompute_chunk
+        net.train_step(*big)
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            net.train_step(*big)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 3
+        out[mode] = {"train_rays_per_s": round(n / dt, 1), "ms_per_step": round(1e3 * dt, 2),
+                     "peak_step_hbm_bytes": int(torch.cuda.max_memory_allocated() - base)}
+        print(f"bench: {prec} {n}-ray train step ({mode}) {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
+        del net
+        torch.cuda.empty_cache()
     return out
 
 

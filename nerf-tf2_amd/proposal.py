@@ -67,6 +67,8 @@ class ProposalNeRF:
         self.optimizer = AdamTF(self.models)
         self.scaler = DynamicLossScaler(self.optimizer) if precision == "bf16" else None
         self.last_fallback_fraction = 0.0
+        self._prop_optimizer = None
+        self._prop_scaler = None
 
     @property
     def models(self):
@@ -92,9 +94,9 @@ class ProposalNeRF:
         missed, _, _ = ray_utils.interlevel(dp["bin_data"]["bin_edges"], pp_p["weights"], df["t_vals"],
                                             pp_f["weights"], far, dilate=self.dilate)
         pp_f["t_vals"] = df["t_vals"]
+        idx = torch.nonzero(missed > self.fallback_threshold).flatten()
+        self.last_fallback_fraction = idx.numel() / max(1, rays_o.shape[0])   # re-rendered, or would be
         if fallback:
-            idx = torch.nonzero(missed > self.fallback_threshold).flatten()
-            self.last_fallback_fraction = idx.numel() / max(1, rays_o.shape[0])
             if idx.numel():
                 o, d = rays_o.index_select(0, idx), rays_d.index_select(0, idx)
                 nr, fr = near.index_select(0, idx), far.index_select(0, idx)
@@ -156,4 +158,33 @@ class ProposalNeRF:
             self.optimizer.apply_gradients(grads, self.scaler)
         return {"fine_loss": lf.detach(), "coarse_loss": lc.detach(), "interlevel": il.mean().detach(),
                 "missed_mean": missed.mean().detach(),
+                "fallback_fraction": (missed > self.fallback_threshold).float().mean().detach()}
+
+    def train_proposal_step(self, rays_o, rays_d, near, far, u_prop=None, u_fine=None) -> Dict[str, Tensor]:
+        """One step of the proposal alone against a fixed fine model (coarse and
+        fine untouched): the histogram upper-bound loss of train_step, with its own
+        Adam state (and loss scale in BF16).  For fitting a proposal to an already
+        trained coarse + fine pair."""
+        if self._prop_optimizer is None:
+            self._prop_optimizer = AdamTF([self.proposal])
+            self._prop_scaler = DynamicLossScaler(self._prop_optimizer) if self.precision == "bf16" else None
+        for mdl in (self.coarse_model, self.fine_model):
+            mdl.params.requires_grad_(False)
+        self.proposal.params.requires_grad_(True)
+        self.proposal.params.grad = None
+        n = rays_o.shape[0]
+        dp, pp_p, df, pp_f = self._proposal_fine(rays_o, rays_d, near, far, u_prop, u_fine)
+        missed, il, g_wp = ray_utils.interlevel(dp["bin_data"]["bin_edges"], pp_p["weights"], df["t_vals"],
+                                                pp_f["weights"].detach(), far, want_grad=True,
+                                                grad_scale=self.interlevel_weight / n, dilate=self.dilate)
+        surrogate = ray_utils.inject_grad(pp_p["weights"], g_wp)
+        if self._prop_scaler is None:
+            surrogate.backward()
+        else:
+            (surrogate * self._prop_scaler.scale_tensor).backward()
+        grad = self.proposal.params.grad
+        grad = grad if grad is not None else torch.zeros_like(self.proposal.params)
+        self.proposal.params.requires_grad_(False)
+        self._prop_optimizer.apply_gradients([grad], self._prop_scaler)
+        return {"interlevel": il.mean().detach(), "missed_mean": missed.mean().detach(),
                 "fallback_fraction": (missed > self.fallback_threshold).float().mean().detach()}

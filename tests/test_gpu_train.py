@@ -198,9 +198,11 @@ def test_train_step_runs_and_learns(nerf):
     assert losses[-1] < 0.5 * losses[0], losses
 
 
-def test_recompute_matches_stash(nerf):
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_recompute_matches_stash(nerf, prec):
     """Chunked activation recomputation gives the same gradients as stashing
-    every activation (only the split-K summation order differs)."""
+    every activation (only the split-K summation order differs; the recomputed
+    bf16 stash, ReLU bits included, is bitwise the one the forward would write)."""
     ro, rd, near, far = _rays_batch(nerf, 1536, seed=13)
     rgb = torch.rand((1536, 3), device=DEV)
     u_c = torch.rand((1536, 64), device=DEV)
@@ -208,7 +210,7 @@ def test_recompute_matches_stash(nerf):
     grads = []
     peaks = []
     for chunk in (None, 512):
-        net = nerf.NeRF(nerf.Params(), "fp32", seed=4)
+        net = nerf.NeRF(nerf.Params(), prec, seed=4)
         for mdl in (net.coarse_model, net.fine_model):
             mdl.recompute_chunk = chunk
             mdl.params.requires_grad_(True)
