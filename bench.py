@@ -58,6 +58,8 @@ def parse():
                     help="train batch of the activation-recomputation measurement (0: skip)")
     ap.add_argument("--recompute-chunk", type=int, default=2048,
                     help="rays per recomputation chunk")
+    ap.add_argument("--no-hbm-kernels", action="store_true",
+                    help="skip the timing of the HBM-bound sampling / compositing kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -122,6 +124,7 @@ def main():
     for form in forms:
         results[form] = measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches)
     head = results[forms[0]]
+    hbm = None if args.no_hbm_kernels else measure_hbm_kernels(nerf, dev)
     recompute = None
     if args.recompute_rays > 0:
         recompute = {p: measure_recompute(p, args, nerf, P, dev, batches) for p in ("fp32", "bf16")}
@@ -151,6 +154,7 @@ def main():
             "avg_launch_ms": head["mlp_avg_launch_ms"], "flop_per_launch": head["mlp_flop_per_launch"]},
         "cpu_baseline": cpu,
         "forms": results,
+        "hbm_kernels": hbm,
         "recompute": recompute,
     }
     if "learned_bf16" in results and "plain_fp32" in results and results["plain_fp32"]["train_rays_per_s"]:
@@ -230,6 +234,63 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
         out["mlp_tflops"] = round(flop / (avg_ms * 1e-3) / 1e12, 2)
     del net
     torch.cuda.empty_cache()
+    return out
+
+
+def measure_hbm_kernels(nerf, dev, n=32768, reps=20):
+    """The HBM-bound kernels of the path at one 32,768-ray render chunk (64 + 128
+    samples), each launched alone through the C ABI and timed with HIP events on
+    its stream: average duration, algorithmic bytes per ray (the float32 inputs it
+    must read and outputs it must write, as called by the render: no materialized
+    xyz, no index outputs) and the achieved rate against the 8 TB/s HBM peak."""
+    import torch
+    L = nerf._lib
+    call, ptr, st = L.call, L.ptr, L.stream(dev)
+    Nc, Nf = 64, 128
+    S = Nc + Nf
+    g = torch.Generator(device=dev).manual_seed(3)
+    f = lambda *shape: torch.rand(shape, device=dev, generator=g)
+    ro = torch.zeros((n, 3), device=dev)
+    rd = torch.nn.functional.normalize(f(n, 3) + 0.1, dim=1)
+    near, far = torch.full((n, 1), 2.0, device=dev), torch.full((n, 1), 6.0, device=dev)
+    u_c, u_f = f(n, Nc), f(n, Nf)
+    edges, widths, t_c = (torch.empty((n, Nc + 1), device=dev), torch.empty((n, Nc), device=dev),
+                          torch.empty((n, Nc), device=dev))
+    w_c = f(n, Nc) ** 4
+    t = torch.empty((n, S), device=dev)
+    sigma, rgb = torch.relu(torch.randn((n * S, 1), device=dev, generator=g)) * 5, f(n * S, 3)
+    w_out, rgb_out, depth, acc = (torch.empty((n, S), device=dev), torch.empty((n, 3), device=dev),
+                                  torch.empty((n,), device=dev), torch.empty((n,), device=dev))
+    g_rgb, g_depth, g_acc = f(n, 3), f(n), f(n)
+    d_sigma, d_rgb = torch.empty((n * S, 1), device=dev), torch.empty((n * S, 3), device=dev)
+    launches = {
+        "coarse_kernel (Nc=64)": (8 + 16 * Nc + 4, lambda: call(
+            "nerf_create_input_batch_coarse", n, Nc, 1, 1, ptr(ro), ptr(rd), ptr(near), ptr(far), ptr(u_c),
+            ptr(edges), ptr(widths), ptr(t_c), None, None, st)),
+        "fine128_kernel (Nc=64, Nf=128)": (4 * (3 * Nc + (Nc + 1) + Nf + S), lambda: call(
+            "nerf_create_input_batch_fine", n, Nc, Nf, ptr(ro), ptr(rd), ptr(w_c), ptr(edges), ptr(widths),
+            ptr(t_c), ptr(u_f), ptr(t), None, None, None, None, st)),
+        "composite_fwd_kernel (S=192)": (24 * S + 20, lambda: call(
+            "nerf_composite_forward", n, S, 1, ptr(sigma), ptr(rgb), ptr(t), ptr(w_out), ptr(rgb_out), ptr(depth),
+            ptr(acc), st)),
+        "composite_bwd_kernel (S=192)": (36 * S + 20, lambda: call(
+            "nerf_composite_backward", n, S, 1, ptr(sigma), ptr(rgb), ptr(t), ptr(g_rgb), ptr(g_depth), ptr(g_acc),
+            None, ptr(d_sigma), ptr(d_rgb), st)),
+    }
+    out = {"rays_per_launch": n}
+    for name, (bpr, fn) in launches.items():
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        gbs = bpr * n / (us * 1e-6) / 1e9
+        out[name] = {"avg_us": round(us, 2), "bytes_per_ray": bpr, "achieved_gbs": round(gbs, 1),
+                     "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 3)}
     return out
 
 

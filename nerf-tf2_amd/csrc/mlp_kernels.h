@@ -285,10 +285,22 @@ __device__ __forceinline__ floatx16 mfma_bf16(bf16x8 a, bf16x8 b, floatx16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
+typedef float floatx2_ __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_ __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_cvt __attribute__((ext_vector_type(4)));
+
+// Two floats -> one packed bf16 pair (round to nearest even): one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((floatx2_){lo, hi}, bf16x2_));
+}
+
+// Converted pair by pair: a whole-vector conversion of values extracted from
+// accumulators is split by the instruction selector into one conversion per
+// element plus byte permutes.
 __device__ __forceinline__ bf16x8 to_bf16x8(float a0, float a1, float a2, float a3, float a4,
                                             float a5, float a6, float a7) {
-  const floatx8 v = {a0, a1, a2, a3, a4, a5, a6, a7};
-  return __builtin_convertvector(v, bf16x8);
+  const u32x4_cvt w = {pack_bf16x2(a0, a1), pack_bf16x2(a2, a3), pack_bf16x2(a4, a5), pack_bf16x2(a6, a7)};
+  return __builtin_bit_cast(bf16x8, w);
 }
 
 // k-step t of a 32-feature block: accumulators 8h .. 8h+7 (h = t % 2).

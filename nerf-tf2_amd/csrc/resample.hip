@@ -140,6 +140,168 @@ __global__ __launch_bounds__(256) void fine_kernel(
   }
 }
 
+// Value of `v` in lane `k` (a compile-time lane after unrolling): v_readlane
+// into a scalar register, no LDS round trip.
+__device__ __forceinline__ float lane_value(float v, int k) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+
+// Count of the first n entries of the ascending row a that are <= x (LE) or
+// < x (!LE): a branch-free binary search with a fixed trip count.
+template <int N, bool LE>
+__device__ __forceinline__ int count_below(const float* a, float x) {
+  int c = 0;
+#pragma unroll
+  for (int step = 1 << (31 - __builtin_clz(N)); step > 0; step >>= 1) {
+    if (c + step <= N) {
+      const float e = a[c + step - 1];
+      if (LE ? (e <= x) : (e < x)) c += step;
+    }
+  }
+  return c;
+}
+
+// The same computation for the shapes of the reference configuration and the
+// learned proposal (n_fine = 128, n_coarse <= 64, each a multiple of 8 or with
+// its Eigen tail), organised for throughput: every cross-lane read of the pdf /
+// cdf passes is a v_readlane of an unrolled, compile-time lane; the searchsorted
+// is a fixed-trip binary search; the merge sorts the 128 fine values with a
+// bitonic network in the wave (2 per lane), places each value by rank (its sorted
+// index + the number of coarse values <= it; a coarse value: its index + the
+// number of fine values < it — coarse first on ties, as the rank merge above), and
+// writes the row from LDS with coalesced stores.  Equal floats are
+// indistinguishable, so the output equals the sorted set bit for bit.
+template <int NC>
+__global__ __launch_bounds__(256) void fine128_kernel(
+    int64_t n_rays, const float* __restrict__ ro, const float* __restrict__ rd,
+    const float* __restrict__ bin_weights, const float* __restrict__ bin_edges,
+    const float* __restrict__ bin_widths, const float* __restrict__ t_c,
+    const float* __restrict__ u_vals, float* __restrict__ t_out, int32_t* __restrict__ idx_out,
+    float* __restrict__ tf_out, float* __restrict__ xyz, float* __restrict__ dirs) {
+  constexpr int NF = 128, S = NC + NF;
+  static_assert(NC >= 2 && NC <= 64, "one coarse bin per lane");
+  __shared__ float s_agg[kWaves][NC + 1];
+  __shared__ float s_pdf[kWaves][NC];
+  __shared__ float s_left[kWaves][NC];
+  __shared__ float s_tc[kWaves][NC];
+  __shared__ float s_sorted[kWaves][NF];
+  __shared__ float s_row[kWaves][S];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t ray = (int64_t)blockIdx.x * kWaves + w;
+  if (ray >= n_rays) return;  // wave-uniform; no block-level barriers below
+
+  // ---- pdf and cdf over the coarse bins (ray_utils.py:335-345)
+  const bool on = lane < NC;
+  const float wv = on ? bin_weights[ray * NC + lane] : 0.f;
+  const float width = on ? bin_widths[ray * NC + lane] : 0.f;
+  const float left = on ? bin_edges[ray * (NC + 1) + lane] : 0.f;
+  const float tc = on ? t_c[ray * NC + lane] : 0.f;
+  const float wp = fadd(wv, 1e-5f);
+  const float prod = fmul(wp, width);
+  constexpr int NV = (NC / 8) * 8;
+  float P[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    P[j] = 0.f;
+#pragma unroll
+    for (int i = j; i < NV; i += 8) P[j] = fadd(P[j], lane_value(prod, i));
+  }
+  float tail = 0.f;
+#pragma unroll
+  for (int i = NV; i < NC; ++i) tail = fadd(tail, lane_value(prod, i));
+  const float denom =
+      fadd(tail, fadd(fadd(fadd(P[0], P[4]), fadd(P[2], P[6])), fadd(fadd(P[1], P[5]), fadd(P[3], P[7]))));
+  const float pdf = fdiv(wp, denom);
+  const float cw = fmul(pdf, width);
+  float agg = 0.f;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {  // sequential inclusive prefix of lane k
+    const float v = lane_value(cw, k);
+    if (k <= lane) agg = fadd(agg, v);
+  }
+  if (lane == 0) s_agg[w][0] = 0.f;
+  if (on) {
+    s_agg[w][lane + 1] = agg;
+    s_pdf[w][lane] = pdf;
+    s_left[w][lane] = left;
+    s_tc[w][lane] = tc;
+  }
+  wave_sync();
+
+  // ---- inverse transform (ray_utils.py:355-377), fine samples lane and lane + 64
+  float v[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int q = lane + 64 * r;
+    const float u = u_vals[ray * NF + q];
+    const int idx = count_below<NC - 1, true>(&s_agg[w][1], u);  // searchsorted(agg[1:-1], u, 'right')
+    const float agg_ = s_agg[w][idx];
+    float pdf_ = s_pdf[w][idx];
+    const float left_ = s_left[w][idx];
+    const float mask = pdf_ < 1e-8f ? 0.f : 1.f;
+    pdf_ = fmaxf(pdf_, 1e-8f);
+    v[r] = fadd(fmul(fdiv(fsub(u, agg_), pdf_), mask), left_);
+    if (idx_out) idx_out[ray * NF + q] = idx;
+    if (tf_out) tf_out[ray * NF + q] = v[r];
+  }
+
+  // ---- bitonic sort of the 128 fine values; element e = lane + 64 r
+#pragma unroll
+  for (int k = 2; k <= NF; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j == 64) {  // partner in the same lane (k == 128: ascending)
+        const float a = v[0], b = v[1];
+        v[0] = fminf(a, b);
+        v[1] = fmaxf(a, b);
+      } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const int e = lane + 64 * r;
+          const bool asc = (e & k) == 0;
+          const float o = shfl_xor(v[r], j);
+          v[r] = (lower == asc) ? fminf(v[r], o) : fmaxf(v[r], o);
+        }
+      }
+    }
+  }
+  s_sorted[w][lane] = v[0];
+  s_sorted[w][lane + 64] = v[1];
+  wave_sync();
+
+  // ---- ranks -> row in LDS -> coalesced stores (ray_utils.py:380-398)
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int e = lane + 64 * r;
+    s_row[w][e + count_below<NC, true>(&s_tc[w][0], v[r])] = v[r];
+  }
+  if (on) s_row[w][lane + count_below<NF, false>(&s_sorted[w][0], tc)] = tc;
+  wave_sync();
+  float o3[3] = {0.f, 0.f, 0.f}, d3[3] = {0.f, 0.f, 0.f};
+  if (xyz) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      o3[c] = ro[3 * ray + c];
+      d3[c] = rd[3 * ray + c];
+    }
+  }
+  float* trow = t_out + ray * S;
+  for (int i = lane; i < S; i += 64) {
+    const float t = s_row[w][i];
+    trow[i] = t;
+    if (xyz) {
+      const int64_t m = ray * S + i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xyz[3 * m + c] = fadd(o3[c], fmul(t, d3[c]));
+        dirs[3 * m + c] = d3[c];
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse, int32_t n_fine,
@@ -161,9 +323,19 @@ extern "C" int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse
   NERF_CHECK_ARG(!xyz_inputs || (rays_o && rays_d),
                  "nerf_create_input_batch_fine: xyz_inputs needs rays");
   if (n_rays == 0) return NERF_OK;
-  fine_kernel<<<(unsigned)((n_rays + kWaves - 1) / kWaves), 64 * kWaves, 0, as_stream(stream)>>>(
-      n_rays, n_coarse, n_fine, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse,
-      u_vals, t_vals, piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
+  const unsigned grid = (unsigned)((n_rays + kWaves - 1) / kWaves);
+  if (n_fine == 128 && n_coarse == 64)
+    fine128_kernel<64><<<grid, 64 * kWaves, 0, as_stream(stream)>>>(
+        n_rays, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse, u_vals, t_vals,
+        piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
+  else if (n_fine == 128 && n_coarse == 48)
+    fine128_kernel<48><<<grid, 64 * kWaves, 0, as_stream(stream)>>>(
+        n_rays, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse, u_vals, t_vals,
+        piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
+  else
+    fine_kernel<<<grid, 64 * kWaves, 0, as_stream(stream)>>>(
+        n_rays, n_coarse, n_fine, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse,
+        u_vals, t_vals, piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
   NERF_LAUNCH_CHECK("nerf_create_input_batch_fine");
   return NERF_OK;
 }

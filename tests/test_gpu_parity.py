@@ -144,12 +144,14 @@ def _oracle_coarse_weights(oracle, n, Nc, seed):
     return w.float()
 
 
-@pytest.mark.parametrize("lin_inv_depth", [True, False])
-def test_resample_bitexact(nerf, oracle, lin_inv_depth):
-    n, Nc, Nf = 1029, 64, 128
+@pytest.mark.parametrize("lin_inv_depth,Nc,Nf", [(True, 64, 128), (False, 64, 128), (True, 48, 128),
+                                                   (True, 40, 96), (False, 64, 200)])
+def test_resample_bitexact(nerf, oracle, lin_inv_depth, Nc, Nf):
+    """(64|48, 128): the register/bitonic resampler; other shapes: the general one."""
+    n = 1029
     ro, rd, near, far = _rays_batch(nerf, n, seed=11)
-    P = nerf.Params(lin_inv_depth=lin_inv_depth)
-    Po = oracle.make_params(lin_inv_depth=lin_inv_depth)
+    P = nerf.Params(N_coarse=Nc, N_fine=Nf, lin_inv_depth=lin_inv_depth)
+    Po = oracle.make_params(N_coarse=Nc, N_fine=Nf, lin_inv_depth=lin_inv_depth)
     u_c = torch.rand((n, Nc), device=DEV)
     dc = nerf.ray_utils.create_input_batch_coarse_model(P, ro, rd, near, far, u_vals=u_c)
     rc = oracle.create_input_batch_coarse_model(Po, ro.cpu(), rd.cpu(), near.cpu(), far.cpu(), u_vals=u_c.cpu())
@@ -158,7 +160,7 @@ def test_resample_bitexact(nerf, oracle, lin_inv_depth):
     _, _, _, agg = oracle.sample_pdf(w, rc["bin_data"]["left_edges"], rc["bin_data"]["bin_widths"],
                                      torch.zeros((n, Nf)))
     u_f = torch.rand((n, Nf))
-    u_f[3, :63] = agg[3, 1:64]                   # exactly on every inner edge
+    u_f[3, :Nc - 1] = agg[3, 1:Nc]               # exactly on every inner edge
     u_f[4, :] = 0.0
     u_f[5, :] = 1.0 - 2.0 ** -24
     u_f[6, :] = agg[6, 1:2].expand(Nf)
