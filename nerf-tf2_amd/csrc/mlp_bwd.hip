@@ -568,6 +568,22 @@ struct ReduceArgs {
   float* grad;
 };
 
+// Sum of n partials p[0], p[stride], ... in sequential order; the loads are
+// issued 16 at a time so that they are in flight together.
+__device__ __forceinline__ float sum_splits(const float* __restrict__ p, int64_t stride, int n) {
+  float s = 0.f;
+  int sp = 0;
+  for (; sp + 16 <= n; sp += 16) {
+    float v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = p[(int64_t)(sp + j) * stride];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) s += v[j];
+  }
+  for (; sp < n; ++sp) s += p[(int64_t)sp * stride];
+  return s;
+}
+
 __global__ void wgrad_reduce_kernel(ReduceArgs a) {
   const int64_t n_el = (int64_t)a.K * a.N;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_el + a.N;
@@ -581,15 +597,13 @@ __global__ void wgrad_reduce_kernel(ReduceArgs a) {
         row = enc_feat(g, s, a.enc_L);
         if (row < 0) continue;
       }
-      float s = 0.f;
-      for (int sp = 0; sp < a.splits; ++sp) s += a.part[(int64_t)sp * n_el + e];
+      const float s = sum_splits(a.part + e, n_el, a.splits);
       float* dst = a.grad + a.k_off + (int64_t)(a.row_base + row) * a.fan_out + n;
       *dst = a.accumulate ? *dst + s : s;
     } else if (a.bpart) {
       const int n = (int)(e - n_el);
       if (n >= a.n_real) continue;
-      float s = 0.f;
-      for (int sp = 0; sp < a.splits; ++sp) s += a.bpart[(int64_t)sp * a.N + n];
+      const float s = sum_splits(a.bpart + n, a.N, a.splits);
       float* dst = a.grad + a.b_off + n;
       *dst = a.accumulate ? *dst + s : s;
     }
@@ -636,6 +650,16 @@ int wgrad_jobs(const MlpPlan& P, WgradJob* jobs) {
 int64_t splits_for(int64_t M) {
   int64_t s = (M + 2047) / 2048;
   if (s > 256) s = 256;
+  return s < 1 ? 1 : s;
+}
+
+// Splits per job of a group of n same-shape bf16 weight-gradient jobs (one
+// launch): 256 workgroups in all, one per CU (each takes most of a CU's LDS),
+// but no split under 256 samples.
+int64_t wg16_splits(int64_t M, int n) {
+  int64_t s = (256 + n - 1) / n;
+  const int64_t cap = (M + 255) / 256;
+  if (s > cap) s = cap;
   return s < 1 ? 1 : s;
 }
 
@@ -707,11 +731,15 @@ extern "C" int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precisio
   return NERF_OK;
 }
 
-// Partial-tile floats of the weight-gradient workspace: room for max(splits,
-// kMaxJobs) tiles of the widest layer (kmax x W) and their bias rows.  A group of
-// n same-shape jobs uses n x max(1, splits / n) <= max(splits, kMaxJobs) tiles.
+// Partial-tile floats of the weight-gradient workspace: room for `s` tiles of the
+// widest layer (kmax x W) and their bias rows, s = the larger of the float32
+// split count and the bf16 bound: a group of n <= kMaxJobs same-shape jobs uses
+// n x wg16_splits(M, n) <= min(256 + kMaxJobs - 1, kMaxJobs x ceil(M / 256)) tiles.
 static int64_t wgrad_workspace_floats(const MlpPlan& P, int64_t M) {
-  const int64_t s = splits_for(M) > kMaxJobs ? splits_for(M) : kMaxJobs;
+  int64_t b16 = kMaxJobs * ((M + 255) / 256);
+  if (b16 > 256 + kMaxJobs - 1) b16 = 256 + kMaxJobs - 1;
+  if (b16 < kMaxJobs) b16 = kMaxJobs;
+  const int64_t s = splits_for(M) > b16 ? splits_for(M) : b16;
   const int64_t kmax = (P.W > 2 * P.KX ? P.W : 2 * P.KX);
   return s * (kmax * P.W + P.W);
 }
@@ -770,7 +798,7 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
           members[n++] = t;
           done[t] = true;
         }
-      const int64_t sg = splits / n > 0 ? splits / n : 1;
+      const int64_t sg = wg16_splits(M, n);
       Wgrad16Args b;
       b.X = reinterpret_cast<const __bf16*>(stash);
       b.ldx = P.stash_per_sample;

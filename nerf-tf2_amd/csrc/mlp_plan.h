@@ -202,7 +202,9 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
     P.st_mask = s;
     s += 16 * D + 8;
   }
-  P.stash_per_sample = ru((int)s, 8);
+  // rows padded to 64 elements: every stash row, and each 64-feature segment a
+  // wave writes, starts on a 128-byte line (bf16), so the forward writes whole lines
+  P.stash_per_sample = ru((int)s, 64);
   // dz (elements per sample: float32 for the fp32 form, bf16 for the bf16 form);
   // every segment starts on a multiple of 8 elements (16-byte aligned bf16 loads)
   int64_t z = 0;
@@ -214,6 +216,6 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
   }
   P.dz_sigma = z; z += 8;
   if (P.rgb) { P.dz_rgb = z; z += 8; }
-  P.dz_per_sample = z;
+  P.dz_per_sample = ru((int)z, 64);  // 128-byte rows (bf16), as the stash
   return 0;
 }

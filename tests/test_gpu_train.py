@@ -27,18 +27,51 @@ def _split(mlp, flat):
 
 def _check_layers(mlp, g_hip, g_ref, tol=1e-3, what=""):
     names = [n for n, _, _ in mlp.config.layer_shapes()]
+    rows, bad = [], []
     for i, (a, b) in enumerate(zip(_split(mlp, g_hip.detach().cpu()), g_ref)):
         b = b.detach().double().reshape(a.shape)
         a = a.double()
         scale = float(b.norm())
         err = float((a - b).norm())
         kind = "kernel" if i % 2 == 0 else "bias"
-        assert err <= tol * max(scale, 1e-12), (f"{what} {names[i // 2]} {kind}: |err|_F {err:.3g} |ref|_F "
-                                                f"{scale:.3g} max err {float((a - b).abs().max()):.3g}")
+        rows.append(f"{what} {names[i // 2]} {kind}: rel {err / max(scale, 1e-30):.2e} (|ref|_F {scale:.3g}, "
+                    f"max err {float((a - b).abs().max()):.3g})")
+        if err > tol * max(scale, 1e-12):
+            bad.append(rows[-1])
+    print("\n".join(rows))
+    assert not bad, "; ".join(bad)
+
+
+def _relu_margin(oracle, conf, W, xyz, dirs):
+    """Per sample, the smallest |pre-activation| over every ReLU of the network,
+    in float64 (the wiring of the oracle's mlp_forward, model.py:355-392)."""
+    W = [w.detach().double() for w in W]
+    enc_x = oracle.positional_encoding(xyz.double(), conf.l_xyz)
+    h, p = enc_x, 0
+    margin = torch.full((xyz.shape[0],), float("inf"), dtype=torch.float64)
+    for i in range(conf.depth):
+        z = h @ W[p] + W[p + 1]
+        p += 2
+        margin = torch.minimum(margin, z.abs().min(dim=1).values)
+        h = torch.relu(z)
+        if i == conf.skip:
+            h = torch.cat([h, enc_x], dim=-1)
+    if conf.l_dir:
+        z = h @ W[p] + W[p + 1]                      # sigma = ReLU(dense)
+        margin = torch.minimum(margin, z.abs().min(dim=1).values)
+        bott = h @ W[p + 2] + W[p + 3]
+        z = torch.cat([bott, oracle.positional_encoding(dirs.double(), conf.l_dir)], dim=-1) @ W[p + 4] + W[p + 5]
+        margin = torch.minimum(margin, z.abs().min(dim=1).values)
+    return margin
 
 
 @pytest.mark.parametrize("cfg", ["nerf", "small", "proposal"])
 def test_mlp_backward(nerf, oracle, cfg):
+    """MLP backward (dX chain + weight GEMMs) against the oracle's autograd at
+    1e-4 (Frobenius, per layer).  Samples with a ReLU pre-activation within 1e-5
+    of zero (float64) get zero upstream gradient on both sides: there the two
+    float32 forwards may take different branches, which is rounding, not the
+    backward under test (about 7 % of the samples of the 8x256 net)."""
     if cfg == "nerf":
         conf, spec = nerf.MLPConfig(), oracle.MLPSpec()
     elif cfg == "small":
@@ -56,6 +89,9 @@ def test_mlp_backward(nerf, oracle, cfg):
     dirs = torch.nn.functional.normalize(torch.randn((M, 3), generator=g), dim=1).float()
     g_rgb = torch.randn((M, 3), generator=g).float()
     g_sig = torch.randn((M, 1), generator=g).float()
+    keep = (_relu_margin(oracle, conf, _oracle_weights(mlp), xyz, dirs) > 1e-5).float()
+    assert float(keep.mean()) > 0.8
+    g_rgb, g_sig = g_rgb * keep[:, None], g_sig * keep[:, None]
     mlp.params.requires_grad_(True)
     mlp.params.grad = None
     if conf.l_dir:
@@ -73,7 +109,7 @@ def test_mlp_backward(nerf, oracle, cfg):
         sig_r = _proposal_forward(oracle, W, xyz)
         Lr = (sig_r * g_sig).sum()
     ref = torch.autograd.grad(Lr, W)
-    _check_layers(mlp, mlp.params.grad, ref, what=cfg)
+    _check_layers(mlp, mlp.params.grad, ref, tol=1e-4, what=cfg)
 
 
 def _proposal_forward(oracle, W, xyz):
@@ -91,9 +127,10 @@ def test_train_step_gradients(nerf, oracle, n):
     same fine sample positions (see test_nerf_forward_config0)."""
     ro, rd, near, far = _rays_batch(nerf, n, seed=21)
     net = nerf.NeRF(nerf.Params(), "fp32", seed=3)
-    rgb = torch.rand((n, 3), device=DEV)
-    u_c = torch.rand((n, 64), device=DEV)
-    u_f = torch.rand((n, 128), device=DEV)
+    gen = torch.Generator(device=DEV).manual_seed(100 + n)   # draws independent of test order
+    rgb = torch.rand((n, 3), device=DEV, generator=gen)
+    u_c = torch.rand((n, 64), device=DEV, generator=gen)
+    u_f = torch.rand((n, 128), device=DEV, generator=gen)
     for mdl in (net.coarse_model, net.fine_model):
         mdl.params.requires_grad_(True)
         mdl.params.grad = None
@@ -204,9 +241,10 @@ def test_recompute_matches_stash(nerf, prec):
     every activation (only the split-K summation order differs; the recomputed
     bf16 stash, ReLU bits included, is bitwise the one the forward would write)."""
     ro, rd, near, far = _rays_batch(nerf, 1536, seed=13)
-    rgb = torch.rand((1536, 3), device=DEV)
-    u_c = torch.rand((1536, 64), device=DEV)
-    u_f = torch.rand((1536, 128), device=DEV)
+    gen = torch.Generator(device=DEV).manual_seed(7)
+    rgb = torch.rand((1536, 3), device=DEV, generator=gen)
+    u_c = torch.rand((1536, 64), device=DEV, generator=gen)
+    u_f = torch.rand((1536, 128), device=DEV, generator=gen)
     grads = []
     peaks = []
     for chunk in (None, 512):
