@@ -12,7 +12,8 @@
 // feeds 64 output features, and no more than one layer's slice is resident.
 //
 // B operands (activations, bf16, in the MFMA B layout) stream from an LDS
-// buffer through a ring of four registers (three loads in flight).  The
+// buffer through a ring of six registers (five loads in flight; four measured
+// 3-4 % slower, eight no better).  The
 // accumulators start from the bias (the first MFMA's C operand is the bias in
 // accumulator layout), so the epilogue of a sample block is: bf16 conversion,
 // ReLU as a packed integer max on the bf16 bits (equal to ReLU before the
@@ -35,6 +36,7 @@ constexpr int kT = 16;           // bf16 k-steps over a 256-wide hidden input
 constexpr int kTX = 4;           // k-steps over the xyz encoding slots (32 per half)
 constexpr int kTD = 2;           // k-steps over the direction encoding slots (16 per half)
 constexpr int kSlab = kT * 64;   // bf16x8 per sample block in one activation buffer
+constexpr int kRing = 6;   // B operands in flight (LDS reads issued 5 steps ahead)
 constexpr int kHalf = 10;        // register k-steps per half (20-step skip layer)
 
 typedef short shortx8 __attribute__((ext_vector_type(8)));
@@ -155,16 +157,17 @@ __device__ __forceinline__ void run_steps(floatx16 (&acc)[kBlocks][2], const bf1
   constexpr bool PAIRS = std::is_same_v<std::decay_t<Epi>, NoEpilogue>;
   auto blk = [](int s) { return PAIRS ? 2 * (s / (2 * H)) + (s & 1) : s / H; };
   auto kst = [](int s) { return PAIRS ? (s >> 1) % H : s % H; };
-  bf16x8 b[4];
+  constexpr int R = kRing;
+  bf16x8 b[R];
 #pragma unroll
-  for (int s = 0; s < 3; ++s) b[s] = b_operand(in, ex, blk(s), T0 + kst(s));
+  for (int s = 0; s < R - 1; ++s) b[s] = b_operand(in, ex, blk(s), T0 + kst(s));
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    if (s + 3 < S) b[(s + 3) & 3] = b_operand(in, ex, blk(s + 3), T0 + kst(s + 3));
+    if (s + R - 1 < S) b[(s + R - 1) % R] = b_operand(in, ex, blk(s + R - 1), T0 + kst(s + R - 1));
     const int sb = blk(s), k = kst(s);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
-      acc[sb][i] = mfma_bf16(w[k][i], b[s & 3], (INIT && k == 0) ? cb[i] : acc[sb][i]);
+      acc[sb][i] = mfma_bf16(w[k][i], b[s % R], (INIT && k == 0) ? cb[i] : acc[sb][i]);
     if (!PAIRS && k == H - 1) epi(sb);
   }
 }
