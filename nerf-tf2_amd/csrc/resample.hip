@@ -12,6 +12,7 @@
 // tf.sort(concat(t_coarse, t_fine)) is a rank merge (coarse values first on
 // ties), so equal values land in equal slots and the output is the sorted set.
 #include "common.h"
+#include <type_traits>
 
 namespace {
 
@@ -146,6 +147,25 @@ __device__ __forceinline__ float lane_value(float v, int k) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
 }
 
+// Lane i - 1's value in lane i, 0 in lane 0 (DPP wave_shr:1, bound_ctrl).
+__device__ __forceinline__ float lane_shift_up(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
+}
+
+// Lane i ^ J's value (J < 64): DPP quad_perm for 1 and 2, row_ror:8 for 8,
+// ds_swizzle (bit mode, within 32 lanes) for 4 and 16, ds_bpermute for 32.
+template <int J>
+__device__ __forceinline__ float lane_xor(float v) {
+  const int x = __builtin_bit_cast(int, v);
+  int r;
+  if constexpr (J == 1) r = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+  else if constexpr (J == 2) r = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+  else if constexpr (J == 8) r = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
+  else if constexpr (J == 4 || J == 16) r = __builtin_amdgcn_ds_swizzle(x, 0x1F | (J << 10));
+  else r = __builtin_bit_cast(int, shfl_xor(v, J));
+  return __builtin_bit_cast(float, r);
+}
+
 // Count of the first n entries of the ascending row a that are <= x (LE) or
 // < x (!LE): a branch-free binary search with a fixed trip count.
 template <int N, bool LE>
@@ -200,13 +220,13 @@ __global__ __launch_bounds__(256) void fine128_kernel(
   const float wp = fadd(wv, 1e-5f);
   const float prod = fmul(wp, width);
   constexpr int NV = (NC / 8) * 8;
+  // lane j < 8: packet sum P_j = prod_j + prod_{j+8} + ... in that order
+  float pj = prod;
+#pragma unroll
+  for (int i = 1; 8 * i < NV; ++i) pj = fadd(pj, shfl_down(prod, 8 * i));
   float P[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    P[j] = 0.f;
-#pragma unroll
-    for (int i = j; i < NV; i += 8) P[j] = fadd(P[j], lane_value(prod, i));
-  }
+  for (int j = 0; j < 8; ++j) P[j] = lane_value(pj, j);
   float tail = 0.f;
 #pragma unroll
   for (int i = NV; i < NC; ++i) tail = fadd(tail, lane_value(prod, i));
@@ -214,12 +234,12 @@ __global__ __launch_bounds__(256) void fine128_kernel(
       fadd(tail, fadd(fadd(fadd(P[0], P[4]), fadd(P[2], P[6])), fadd(fadd(P[1], P[5]), fadd(P[3], P[7]))));
   const float pdf = fdiv(wp, denom);
   const float cw = fmul(pdf, width);
-  float agg = 0.f;
+  // sequential inclusive prefix: after step s lane i holds cw_{i-s} + ... + cw_i added
+  // left to right (lane -1 reads 0), so after NC - 1 steps every lane < NC holds
+  // ((cw_0 + cw_1) + ...) + cw_i exactly
+  float agg = cw;
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {  // sequential inclusive prefix of lane k
-    const float v = lane_value(cw, k);
-    if (k <= lane) agg = fadd(agg, v);
-  }
+  for (int step = 1; step < NC; ++step) agg = fadd(lane_shift_up(agg), cw);
   if (lane == 0) s_agg[w][0] = 0.f;
   if (on) {
     s_agg[w][lane + 1] = agg;
@@ -247,26 +267,40 @@ __global__ __launch_bounds__(256) void fine128_kernel(
   }
 
   // ---- bitonic sort of the 128 fine values; element e = lane + 64 r
+  auto stage = [&](auto kc, auto jc) {
+    constexpr int k = decltype(kc)::value, j = decltype(jc)::value;
+    if constexpr (j == 64) {  // partner in the same lane (k == 128: ascending)
+      const float a = v[0], b = v[1];
+      v[0] = fminf(a, b);
+      v[1] = fmaxf(a, b);
+    } else {
+      const bool lower = (lane & j) == 0;
 #pragma unroll
-  for (int k = 2; k <= NF; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j == 64) {  // partner in the same lane (k == 128: ascending)
-        const float a = v[0], b = v[1];
-        v[0] = fminf(a, b);
-        v[1] = fmaxf(a, b);
-      } else {
-        const bool lower = (lane & j) == 0;
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          const int e = lane + 64 * r;
-          const bool asc = (e & k) == 0;
-          const float o = shfl_xor(v[r], j);
-          v[r] = (lower == asc) ? fminf(v[r], o) : fmaxf(v[r], o);
-        }
+      for (int r = 0; r < 2; ++r) {
+        const int e = lane + 64 * r;
+        const bool asc = (e & k) == 0;
+        const float o = lane_xor<j>(v[r]);
+        v[r] = (lower == asc) ? fminf(v[r], o) : fmaxf(v[r], o);
       }
     }
-  }
+  };
+  auto merge = [&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if constexpr (k >= 128) stage(kc, std::integral_constant<int, 64>{});
+    if constexpr (k >= 64) stage(kc, std::integral_constant<int, 32>{});
+    if constexpr (k >= 32) stage(kc, std::integral_constant<int, 16>{});
+    if constexpr (k >= 16) stage(kc, std::integral_constant<int, 8>{});
+    if constexpr (k >= 8) stage(kc, std::integral_constant<int, 4>{});
+    if constexpr (k >= 4) stage(kc, std::integral_constant<int, 2>{});
+    stage(kc, std::integral_constant<int, 1>{});
+  };
+  merge(std::integral_constant<int, 2>{});
+  merge(std::integral_constant<int, 4>{});
+  merge(std::integral_constant<int, 8>{});
+  merge(std::integral_constant<int, 16>{});
+  merge(std::integral_constant<int, 32>{});
+  merge(std::integral_constant<int, 64>{});
+  merge(std::integral_constant<int, 128>{});
   s_sorted[w][lane] = v[0];
   s_sorted[w][lane + 64] = v[1];
   wave_sync();
