@@ -191,7 +191,11 @@ int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precision, const vo
 /* Weight gradients: grad_flat (param_count floats, flat layout) = or +=
  * (accumulate != 0) sum over the M samples of X^T dz for every layer, and the
  * bias sums.  workspace: nerf_mlp_wgrad_workspace_bytes(desc, M) bytes.
- * The summation order is fixed (deterministic, no float atomics). */
+ * The summation order is fixed (deterministic, no float atomics).
+ * M == 0 (an empty batch): grad_flat is zeroed (or left as is when
+ * accumulating); stash, dz and workspace may then be NULL.  Every other entry
+ * point returns NERF_OK without reading its pointers when its ray / sample
+ * count is 0. */
 int64_t nerf_mlp_wgrad_workspace_bytes(const nerf_mlp_desc* desc, int64_t M);
 int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
                               const void* stash, const void* dz, float* grad_flat,

@@ -221,6 +221,7 @@ int32_t launch_fwd(int64_t n_rays, int S, int white_bg, const T* sigma, const T*
                    void* stream, const char* name) {
   NERF_CHECK_ARG(n_rays >= 0 && S >= 1 && S <= 64 * kMaxP, "%s: S=%d must be in [1, %d]", name,
                  S, 64 * kMaxP);
+  if (n_rays == 0) return NERF_OK;  // empty batch: nothing to read or write
   NERF_CHECK_ARG(sigma && t_vals && weights && pred_rgb && pred_depth && acc_map,
                  "%s: null pointer", name);
   if (n_rays == 0) return NERF_OK;
@@ -253,6 +254,7 @@ int32_t launch_bwd(int64_t n_rays, int S, int white_bg, const T* sigma, const T*
                    const T* g_w, T* d_sigma, T* d_rgb, void* stream, const char* name) {
   NERF_CHECK_ARG(n_rays >= 0 && S >= 1 && S <= 64 * kMaxP, "%s: S=%d must be in [1, %d]", name,
                  S, 64 * kMaxP);
+  if (n_rays == 0) return NERF_OK;
   NERF_CHECK_ARG(sigma && t_vals && g_rgb && d_sigma && (d_rgb || !rgb), "%s: null pointer", name);
   if (n_rays == 0) return NERF_OK;
   const int P = (S + 63) / 64;

@@ -568,6 +568,11 @@ struct ReduceArgs {
   float* grad;
 };
 
+__global__ void zero_kernel(float* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
 // Sum of n partials p[0], p[stride], ... in sequential order; the loads are
 // issued 16 at a time so that they are in flight together.
 __device__ __forceinline__ float sum_splits(const float* __restrict__ p, int64_t stride, int n) {
@@ -721,7 +726,9 @@ extern "C" int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precisio
   MlpPlan P;
   NERF_CHECK_ARG(nerf_plan_from_desc(desc, precision, &P) == 0,
                  "nerf_mlp_backward: unsupported MLP description or precision");
-  NERF_CHECK_ARG(M >= 0 && packed && d_sigma && sigma && stash && dz,
+  NERF_CHECK_ARG(M >= 0, "nerf_mlp_backward: M < 0");
+  if (M == 0) return NERF_OK;  // empty batch
+  NERF_CHECK_ARG(packed && d_sigma && sigma && stash && dz,
                  "nerf_mlp_backward: null pointer");
   NERF_CHECK_ARG(!P.rgb || (d_rgb && rgb), "nerf_mlp_backward: d_rgb and rgb required");
   if (M == 0) return NERF_OK;
@@ -756,9 +763,16 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
   MlpPlan P;
   NERF_CHECK_ARG(nerf_plan_from_desc(desc, precision, &P) == 0,
                  "nerf_mlp_weight_grads: unsupported MLP description or precision");
-  NERF_CHECK_ARG(M > 0 && stash && dz && grad_flat && workspace,
-                 "nerf_mlp_weight_grads: bad arguments");
+  NERF_CHECK_ARG(M >= 0 && grad_flat, "nerf_mlp_weight_grads: bad arguments");
   hipStream_t s = as_stream(stream);
+  if (M == 0) {  // empty batch: the gradient is zero (a fill kernel, no memset node in a graph)
+    if (!accumulate)
+      zero_kernel<<<(unsigned)((P.n_params + 255) / 256 < 4096 ? (P.n_params + 255) / 256 : 4096), 256, 0, s>>>(
+          grad_flat, P.n_params);
+    NERF_LAUNCH_CHECK("nerf_mlp_weight_grads");
+    return NERF_OK;
+  }
+  NERF_CHECK_ARG(stash && dz && workspace, "nerf_mlp_weight_grads: bad arguments");
   WgradJob jobs[2 * NERF_MAX_LAYERS];
   const int nj = wgrad_jobs(P, jobs);
   const int64_t splits = splits_for(M);

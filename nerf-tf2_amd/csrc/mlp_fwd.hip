@@ -136,6 +136,7 @@ extern "C" int32_t nerf_mlp_forward(const nerf_mlp_desc* desc, int32_t precision
   NERF_CHECK_ARG(nerf_plan_from_desc(desc, precision, &P) == 0,
                  "nerf_mlp_forward: unsupported MLP description or precision");
   NERF_CHECK_ARG(M >= 0, "nerf_mlp_forward: M < 0");
+  if (M == 0) return NERF_OK;  // empty batch
   NERF_CHECK_ARG(packed && sigma, "nerf_mlp_forward: null pointer");
   NERF_CHECK_ARG(!P.rgb || rgb, "nerf_mlp_forward: rgb output required for this MLP");
   if (rays_o) {

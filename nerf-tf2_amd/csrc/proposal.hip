@@ -95,6 +95,7 @@ extern "C" int32_t nerf_proposal_interlevel(int64_t n_rays, int32_t n_prop, int3
   NERF_CHECK_ARG(n_rays >= 0 && n_prop >= 1 && n_prop <= kMaxSp && n_fine >= 1 && n_fine <= kMaxSf,
                  "nerf_proposal_interlevel: n_prop=%d (1..%d) n_fine=%d (1..%d)", n_prop, kMaxSp,
                  n_fine, kMaxSf);
+  if (n_rays == 0) return NERF_OK;  // empty batch
   NERF_CHECK_ARG(prop_edges && prop_weights && t_fine && fine_weights && far,
                  "nerf_proposal_interlevel: null pointer");
   NERF_CHECK_ARG(dilate >= 0 && dilate < n_prop, "nerf_proposal_interlevel: dilate=%d", dilate);

@@ -350,6 +350,7 @@ extern "C" int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse
                      n_fine <= kMaxNf,
                  "nerf_create_input_batch_fine: n_coarse=%d (2..%d) n_fine=%d (1..%d)", n_coarse,
                  kMaxNc, n_fine, kMaxNf);
+  if (n_rays == 0) return NERF_OK;  // empty batch
   NERF_CHECK_ARG(bin_weights && bin_edges && bin_widths && t_vals_coarse && u_vals && t_vals,
                  "nerf_create_input_batch_fine: null pointer");
   NERF_CHECK_ARG((xyz_inputs == nullptr) == (dir_inputs == nullptr),

@@ -170,6 +170,7 @@ extern "C" int32_t nerf_create_input_batch_coarse(int64_t n_rays, int32_t n_coar
                                                   float* xyz_inputs, float* dir_inputs,
                                                   void* stream) {
   NERF_CHECK_ARG(n_rays >= 0 && n_coarse > 0, "nerf_create_input_batch_coarse: bad sizes");
+  if (n_rays == 0) return NERF_OK;  // empty batch
   NERF_CHECK_ARG(near && far && bin_edges && bin_widths && t_vals,
                  "nerf_create_input_batch_coarse: null pointer");
   NERF_CHECK_ARG(!perturb || u_vals, "nerf_create_input_batch_coarse: perturb needs u_vals");
@@ -232,6 +233,7 @@ extern "C" int32_t nerf_sample_batch(int64_t batch, int32_t n_images, int32_t H,
                                      const int64_t* pixel_idx, float* rays_o, float* rays_d,
                                      float* near, float* far, float* rgb, void* stream) {
   NERF_CHECK_ARG(batch >= 0 && n_images > 0 && H > 0 && W > 0, "nerf_sample_batch: bad sizes");
+  if (batch == 0) return NERF_OK;
   NERF_CHECK_ARG(images && poses && Ks && bounds && image_idx && pixel_idx && rays_o && rays_d && near &&
                      far && rgb,
                  "nerf_sample_batch: null pointer");

@@ -425,7 +425,7 @@ class NeRF:
         outs = {"pred_rgb": [], "pred_depth": [], "acc_map": []}
         if keep_weights:
             outs["weights"] = []
-        for i in range(0, rays_o.shape[0], chunk):
+        for i in range(0, max(rays_o.shape[0], 1), chunk):   # an empty batch runs once, empty
             _, pp_f = self.forward(rays_o[i:i + chunk], rays_d[i:i + chunk], near[i:i + chunk], far[i:i + chunk])
             for k in outs:
                 outs[k].append(pp_f[k])

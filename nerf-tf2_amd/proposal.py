@@ -110,7 +110,7 @@ class ProposalNeRF:
     def predict(self, rays_o, rays_d, near, far, chunk: int = 32768, fallback: bool = True):
         outs = {"pred_rgb": [], "pred_depth": [], "acc_map": []}
         n_fb = 0
-        for i in range(0, rays_o.shape[0], chunk):
+        for i in range(0, max(rays_o.shape[0], 1), chunk):   # an empty batch runs once, empty
             _, pp_f, _ = self.forward(rays_o[i:i + chunk], rays_d[i:i + chunk], near[i:i + chunk],
                                       far[i:i + chunk], fallback=fallback)
             n_fb += int(round(self.last_fallback_fraction * min(chunk, rays_o.shape[0] - i)))
