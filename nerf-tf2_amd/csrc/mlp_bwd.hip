@@ -589,10 +589,11 @@ __device__ __forceinline__ float sum_splits(const float* __restrict__ p, int64_t
   return s;
 }
 
-__global__ void wgrad_reduce_kernel(ReduceArgs a) {
+// Blocks blk (of nblk) of the reduction of one job: element e of the K x N tile
+// (and of the bias row) summed over the splits into the flat gradient.
+__device__ __forceinline__ void reduce_job(const ReduceArgs& a, int64_t blk, int64_t nblk) {
   const int64_t n_el = (int64_t)a.K * a.N;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_el + a.N;
-       e += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t e = blk * blockDim.x + threadIdx.x; e < n_el + a.N; e += nblk * blockDim.x) {
     if (e < n_el) {
       const int k = (int)(e / a.N), n = (int)(e % a.N);
       if (n >= a.n_real) continue;
@@ -613,6 +614,22 @@ __global__ void wgrad_reduce_kernel(ReduceArgs a) {
       *dst = a.accumulate ? *dst + s : s;
     }
   }
+}
+
+__global__ void wgrad_reduce_kernel(ReduceArgs a) { reduce_job(a, blockIdx.x, gridDim.x); }
+
+// The reductions of a group of jobs in one launch: job j owns blocks
+// [first[j], first[j + 1]).
+struct ReduceBatch {
+  ReduceArgs job[kMaxJobs];
+  int first[kMaxJobs + 1];
+  int njobs;
+};
+
+__global__ void wgrad_reduce_batch_kernel(ReduceBatch b) {
+  int j = 0;
+  while (j + 1 < b.njobs && (int)blockIdx.x >= b.first[j + 1]) ++j;
+  reduce_job(b.job[j], (int64_t)blockIdx.x - b.first[j], b.first[j + 1] - b.first[j]);
 }
 
 // One (X-segment, Z) GEMM + reduction into the flat gradient.
@@ -778,7 +795,7 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
   const int64_t splits = splits_for(M);
   const int64_t mps = (M + splits - 1) / splits;
   float* part = reinterpret_cast<float*>(workspace);
-  auto reduce = [&](const WgradJob& J, const float* jp, const float* jb, int nsplit) {
+  auto reduce_args = [&](const WgradJob& J, const float* jp, const float* jb, int nsplit) {
     ReduceArgs r;
     r.part = jp;
     r.bpart = jb;
@@ -794,10 +811,15 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
     r.fan_out = P.fan_out[J.keras];
     r.accumulate = accumulate;
     r.grad = grad_flat;
+    return r;
+  };
+  auto reduce_blocks = [](const WgradJob& J) {
     const int64_t tot = (int64_t)J.K * J.N + J.N;
-    int64_t rb = (tot + 255) / 256;
-    if (rb > 1024) rb = 1024;
-    wgrad_reduce_kernel<<<(unsigned)rb, 256, 0, s>>>(r);
+    const int64_t rb = (tot + 255) / 256;
+    return (int)(rb > 1024 ? 1024 : rb);
+  };
+  auto reduce = [&](const WgradJob& J, const float* jp, const float* jb, int nsplit) {
+    wgrad_reduce_kernel<<<(unsigned)reduce_blocks(J), 256, 0, s>>>(reduce_args(J, jp, jb, nsplit));
   };
   if (precision == NERF_PREC_BF16) {
     // group the jobs by tile shape; each group is one launch with splits / njobs splits per job
@@ -836,7 +858,14 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
       }
       NERF_CHECK_ARG(wp - part <= wgrad_workspace_floats(P, M), "nerf_mlp_weight_grads: workspace layout");
       launch_wgrad_bf16(b, kb, nb, s);
-      for (int i = 0; i < n; ++i) reduce(jobs[members[i]], b.job[i].part, b.job[i].bpart, (int)sg);
+      ReduceBatch rbt;
+      rbt.njobs = n;
+      rbt.first[0] = 0;
+      for (int i = 0; i < n; ++i) {
+        rbt.job[i] = reduce_args(jobs[members[i]], b.job[i].part, b.job[i].bpart, (int)sg);
+        rbt.first[i + 1] = rbt.first[i] + reduce_blocks(jobs[members[i]]);
+      }
+      wgrad_reduce_batch_kernel<<<(unsigned)rbt.first[n], 256, 0, s>>>(rbt);
     }
   } else {
     for (int t = 0; t < nj; ++t) {
