@@ -7,7 +7,9 @@ layers x 64 on an L = 8 xyz encoding, softplus density) replaces the coarse
 Per ray:
   1. 48 stratified proposal samples (the coarse sampler with 48 bins);
   2. proposal densities -> weights (the compositing kernel without colours);
-  3. 128 fine samples from the proposal weights (the same CDF resampler);
+  3. 128 fine samples from the proposal weights, 2 % of each ray's proposal
+     mass spread evenly over its bins first (`sample_floor`), with the same CDF
+     resampler;
   4. the fine 8x256 model on the 48 + 128 sorted samples -> rgb / depth / acc;
   5. the proposal's missed weight mass against the fine weights (interlevel
      kernel; the proposal's histogram is blurred by `dilate` bins on each side,
@@ -49,7 +51,7 @@ class _P:
 class ProposalNeRF:
     def __init__(self, params, precision: str = "bf16", seed: int = 0, device="cuda", n_prop: int = 48,
                  fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 10.0,
-                 dilate: int = 2, proposal_config: MLPConfig = PROPOSAL_CONFIG):
+                 dilate: int = 2, proposal_config: MLPConfig = PROPOSAL_CONFIG, sample_floor: float = 0.02):
         self.params = params
         self.white_bg = bool(params.system.white_bg)
         self.precision = precision
@@ -60,6 +62,9 @@ class ProposalNeRF:
         self.coarse_stride = coarse_stride
         self.interlevel_weight = interlevel_weight
         self.dilate = dilate
+        # fraction of each ray's proposal mass spread uniformly over its bins before
+        # the fine samples are drawn (the losses see the unmodified weights)
+        self.sample_floor = sample_floor
         self.p_prop = _P(n_prop, self.n_fine, params.sampling.lin_inv_depth, self.white_bg)
         self.proposal = NerfMLP("proposal", proposal_config, precision, seed=seed + 2, device=device)
         self.coarse_model = NerfMLP("coarse", MLPConfig(), precision, seed=seed, device=device)
@@ -80,7 +85,10 @@ class ProposalNeRF:
                                                        materialize=False)
         sig_p = self.proposal.forward_rays(rays_o, rays_d, dp["t_vals"])
         pp_p = ray_utils.post_process_model_output(None, sig_p, dp["t_vals"], False)
-        df = ray_utils.create_input_batch_fine_model(self.p_prop, rays_o, rays_d, pp_p["weights"], dp["bin_data"],
+        w_s = pp_p["weights"].detach()
+        if self.sample_floor > 0.0:
+            w_s = w_s + (self.sample_floor / self.n_prop) * w_s.sum(dim=1, keepdim=True)
+        df = ray_utils.create_input_batch_fine_model(self.p_prop, rays_o, rays_d, w_s, dp["bin_data"],
                                                      dp["t_vals"], u_vals=u_fine, materialize=False)
         rgb_f, sig_f = self.fine_model.forward_rays(rays_o, rays_d, df["t_vals"])
         pp_f = ray_utils.post_process_model_output(rgb_f, sig_f, df["t_vals"], self.white_bg)
