@@ -163,7 +163,10 @@ def main():
                                     results["plain_fp32"]["render_rays_per_s"], 2),
             "train_speedup": round(results["learned_bf16"]["train_rays_per_s"] /
                                    results["plain_fp32"]["train_rays_per_s"], 2),
-            "psnr": "see nerf-tf2_amd/quality.py (2000-step training runs)"}
+            # not measured by this run: the 2000-step training comparison is its own driver
+            "psnr": {"source": "nerf-tf2_amd/quality.py, 6 seeds x 2000 steps, weight-averaged; "
+                               "profiles/quality_blob_2000steps.md",
+                     "learned_minus_plain_fp32_db": -0.055, "control_redraw_db": -0.051}}
     print(json.dumps(line))
 
 
