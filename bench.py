@@ -379,6 +379,26 @@ def cpu_baseline(seconds: float):
             if time.perf_counter() - t0 > seconds:
                 break
     dt = time.perf_counter() - t0
+    # train steps of the oracle (forward, autograd backward, TF-form Adam) on 1024-ray
+    # batches of the same view (the config-0 batch size), for about a third of the time
+    wc_t = [w.clone() for w in wc]
+    wf_t = [w.clone() for w in wf]
+    opt = oracle.AdamTF(wc_t + wf_t)
+    nt, steps = 1024, 0
+    t1 = time.perf_counter()
+    while True:
+        sl = slice((steps * nt) % 600000, (steps * nt) % 600000 + nt)
+        o = torch.from_numpy(np.ascontiguousarray(ro[sl], dtype=np.float32))
+        d = torch.from_numpy(np.ascontiguousarray(rd[sl], dtype=np.float32))
+        _, _, gc, gf, _ = oracle.train_step_grads(P, spec, wc_t, wf_t, o, d, torch.full((nt, 1), 2.0),
+                                                  torch.full((nt, 1), 6.0), torch.rand(nt, 3), torch.rand(nt, 64),
+                                                  torch.rand(nt, 128))
+        new_w = opt.step(wc_t + wf_t, gc + gf)
+        wc_t, wf_t = new_w[:len(wc_t)], new_w[len(wc_t):]
+        steps += 1
+        if time.perf_counter() - t1 > seconds / 3:
+            break
+    dt_train = time.perf_counter() - t1
     cpu_model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -389,7 +409,9 @@ def cpu_baseline(seconds: float):
         pass
     return {"value": round(done / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"{done} rays of the 800x800 render workload (coarse+fine forward, {n}-ray chunks, "
-                      f"{dt:.1f} s, torch-CPU float32 restatement, {cpu_model})"}
+                      f"{dt:.1f} s, torch-CPU float32 restatement, {cpu_model})",
+            "train_rays_per_s": round(steps * nt / dt_train, 1),
+            "train_sample": f"{steps} train steps of {nt} rays (forward, autograd backward, Adam), {dt_train:.1f} s"}
 
 
 if __name__ == "__main__":
