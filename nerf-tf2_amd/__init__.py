@@ -7,7 +7,7 @@ a hyphen).  Modules:
   scene      the procedural Gaussian-blob dataset (float64 ground truth)
   _lib       ctypes binding of libnerf_hip.so (include/nerf_hip.h)
 """
-from . import _lib, ray_utils, model, scene, graphs, render  # noqa: F401
+from . import _lib, ray_utils, model, scene, graphs, render, checkpoint  # noqa: F401
 from .model import MLPConfig, NeRF, NerfMLP, AdamTF, DynamicLossScaler, get_coarse_or_fine_model  # noqa: F401
 from .proposal import ProposalNeRF  # noqa: F401
 from .graphs import GraphedTrainStep  # noqa: F401

@@ -68,3 +68,17 @@ def test_invalid_arguments_rejected_without_launch(nerf):
     assert b"n_coarse=65" in lib.nerf_last_error()
     rc = lib.nerf_composite_forward(10, 1000, 1, *([None] * 7), None)
     assert rc == nerf._lib.NERF_ERR_INVALID_ARG
+
+
+def test_checkpoint_variable_layout(nerf):
+    """Keras variable names and shapes of the optimizer / weight files
+    (model.py:355-387): 24 variables per model, 595,844 values."""
+    v = nerf.checkpoint.keras_variables(nerf.MLPConfig(), "coarse")
+    assert len(v) == 24
+    assert v[0] == ("coarse/dense_0/kernel", (63, 256)) and v[1] == ("coarse/dense_0/bias", (256,))
+    assert v[10] == ("coarse/dense_5/kernel", (319, 256))
+    assert [n for n, _ in v[16:]] == ["coarse/sigma/kernel", "coarse/sigma/bias", "coarse/dense_8/kernel",
+                                      "coarse/dense_8/bias", "coarse/dense_9/kernel", "coarse/dense_9/bias",
+                                      "coarse/rgb/kernel", "coarse/rgb/bias"]
+    assert v[20][1] == (283, 128)
+    assert sum(int(np.prod(s)) for _, s in v) == 595844
