@@ -4,7 +4,7 @@ The reference's SaveCallback (nerf/core/ops.py:98-150) writes, per checkpoint
 tag, `{tag}_coarse.h5` and `{tag}_fine.h5` (Keras `model.save_weights`) and, if
 enabled, `{tag}_optimizer.npz`: every variable of the Keras Adam optimizer by
 name plus a `names` array holding their order (ops.py:110-120).
-NeRF.load_everything (nerf/core/model.py:245-287) reads them back, requiring
+NeRF.set_everything (nerf/core/model.py:245-287) reads them back, requiring
 the optimizer's variable names to equal the saved `names`.
 
 * Optimizer state: save_optimizer_npz / load_optimizer_npz write and read that
