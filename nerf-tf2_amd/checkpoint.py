@@ -24,7 +24,7 @@ the optimizer's variable names to equal the saved `names`.
 """
 from __future__ import annotations
 
-from typing import Dict, List, Sequence, Tuple
+from typing import Dict, List, Tuple
 
 import numpy as np
 import torch
@@ -53,13 +53,6 @@ def _split(mlp: NerfMLP, flat: torch.Tensor) -> List[np.ndarray]:
         out.append(arr[off:off + n].reshape(shape))
         off += n
     return out
-
-
-def _join(mlp: NerfMLP, arrays: Sequence[np.ndarray]) -> torch.Tensor:
-    flat = np.concatenate([np.asarray(a, dtype=np.float32).reshape(-1) for a in arrays])
-    if flat.size != mlp.n_params:
-        raise ValueError(f"{mlp.name}: expected {mlp.n_params} values, got {flat.size}")
-    return torch.from_numpy(flat).to(mlp.device)
 
 
 def save_weights_npz(path: str, mlp: NerfMLP) -> None:
