@@ -685,6 +685,14 @@ int64_t wg16_splits(int64_t M, int n) {
   return s < 1 ? 1 : s;
 }
 
+// Fewest splits that keep one split's rows (and the 256 rows of zero padding a
+// chunk may read past them) under 2^31 bytes: a split's buffer resources
+// address its rows with 32-bit offsets.
+int64_t wg16_min_splits(int64_t M, int64_t row_bytes) {
+  const int64_t lim = (((int64_t)1 << 31) - 1) / row_bytes - 256;
+  return (M + lim - 1) / lim;
+}
+
 template <int W, int D, int SKIP, int LX, int LD>
 int launch_bwd_f32(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,
                    const float* d_sigma, const float* rgb, const float* sigma, const void* stash,
@@ -763,6 +771,8 @@ static int64_t wgrad_workspace_floats(const MlpPlan& P, int64_t M) {
   int64_t b16 = kMaxJobs * ((M + 255) / 256);
   if (b16 > 256 + kMaxJobs - 1) b16 = 256 + kMaxJobs - 1;
   if (b16 < kMaxJobs) b16 = kMaxJobs;
+  const int64_t row = 2 * (P.stash_per_sample > P.dz_per_sample ? P.stash_per_sample : P.dz_per_sample);
+  if (b16 < kMaxJobs * wg16_min_splits(M, row)) b16 = kMaxJobs * wg16_min_splits(M, row);
   const int64_t s = splits_for(M) > b16 ? splits_for(M) : b16;
   const int64_t kmax = (P.W > 2 * P.KX ? P.W : 2 * P.KX);
   return s * (kmax * P.W + P.W);
@@ -834,7 +844,9 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
           members[n++] = t;
           done[t] = true;
         }
-      const int64_t sg = wg16_splits(M, n);
+      const int64_t row_bytes = 2 * (P.stash_per_sample > P.dz_per_sample ? P.stash_per_sample : P.dz_per_sample);
+      int64_t sg = wg16_splits(M, n);
+      if (sg < wg16_min_splits(M, row_bytes)) sg = wg16_min_splits(M, row_bytes);
       Wgrad16Args b;
       b.X = reinterpret_cast<const __bf16*>(stash);
       b.ldx = P.stash_per_sample;

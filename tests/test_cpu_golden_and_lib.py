@@ -53,6 +53,10 @@ def test_size_queries(nerf):
     for M in (1, 100, 4099, 2048 * 5):
         assert lib.nerf_mlp_wgrad_workspace_bytes(ctypes.byref(d), M) >= 8 * tile
     assert lib.nerf_mlp_wgrad_workspace_bytes(ctypes.byref(d), 10 ** 7) >= 256 * tile
+    # huge batches: more splits, so that one split's rows stay under 2^31 bytes (32-bit buffer offsets)
+    row = 2 * max(lib.nerf_mlp_stash_floats_per_sample(ctypes.byref(d)), lib.nerf_mlp_dz_floats_per_sample(ctypes.byref(d)))
+    min_splits = -(-10 ** 9 * row // 2 ** 31)
+    assert lib.nerf_mlp_wgrad_workspace_bytes(ctypes.byref(d), 10 ** 9) >= 8 * min_splits * tile
     # dz and stash rows start every segment on 8 elements (16-byte bf16 row chunks)
     assert lib.nerf_mlp_dz_floats_per_sample(ctypes.byref(d)) % 8 == 0
     assert lib.nerf_mlp_stash_floats_per_sample(ctypes.byref(d)) % 8 == 0
