@@ -539,11 +539,155 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   }
 }
 
+// The 256 x 256 tile with 8 waves (two per SIMD): wave w owns output column block
+// w for all 8 row blocks (128 accumulators), which leaves registers for two
+// chunks of loads in flight per thread (chunk k is loaded into register set
+// k % 2 two iterations before it goes to LDS).  Same chunking, transposed
+// XOR-swizzled LDS tiles and fixed-order reductions as wgrad_bf16_kernel.
+__global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
+  constexpr int NFG = 32, R = 2, PP = 512 / NFG;  // feature groups, rounds, sample pairs per round
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kWF][kWP];
+  __shared__ __attribute__((aligned(16))) __bf16 Zt[2][kWF][kWP];
+  const int split = blockIdx.x % a.splits;
+  const Wgrad16Job J = a.job[blockIdx.x / a.splits];
+  const int64_t mb = (int64_t)split * a.m_per_split;
+  const int64_t me = mb + a.m_per_split < a.M ? mb + a.m_per_split : a.M;
+  const int64_t rows = me > mb ? me - mb : 0;
+  const int t = threadIdx.x, lane = t & 63, r = lane & 31, q = lane >> 5;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int fg = t % NFG, ps = t / NFG;
+  const int sw = 4 * ((fg >> 1) & 7);
+  const int qx = q ^ (r >> 4);
+  const __amdgpu_buffer_rsrc_t rx = rows_rsrc(a.X + mb * a.ldx, rows * a.ldx * 2);
+  const __amdgpu_buffer_rsrc_t rz = rows_rsrc(a.Z + mb * a.ldz, rows * a.ldz * 2);
+
+  floatx16 acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[i][e] = 0.f;
+  float bsum[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
+
+  u32x4_ xr[2][R][2], zr[2][R][2];
+  // per-thread byte offsets; the row of each load (chunk, round, pair half) is added
+  // to the VGPR offset, which the buffer range check covers (rows past the split
+  // read zeros), not to the unchecked scalar offset
+  const uint32_t vx = (uint32_t)(((2 * ps) * a.ldx + J.xoff + 8 * fg) * 2);
+  const uint32_t vz = (uint32_t)(((2 * ps) * a.ldz + J.zoff + 8 * fg) * 2);
+  auto load_chunk = [&](auto set, int c) {
+    constexpr int S = decltype(set)::value;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t row = (uint32_t)(c * kWC + 2 * PP * i + h);
+        xr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(rx, vx + row * (uint32_t)(a.ldx * 2), 0, 0);
+        zr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(rz, vz + row * (uint32_t)(a.ldz * 2), 0, 0);
+      }
+  };
+  auto store_chunk = [&](auto set, int buf) {
+    constexpr int S = decltype(set)::value;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int col = 2 * ((ps + PP * i) ^ sw);
+      const bf16x8 x0 = __builtin_bit_cast(bf16x8, xr[S][i][0]), x1 = __builtin_bit_cast(bf16x8, xr[S][i][1]);
+      const bf16x8 z0 = __builtin_bit_cast(bf16x8, zr[S][i][0]), z1 = __builtin_bit_cast(bf16x8, zr[S][i][1]);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        *reinterpret_cast<bf16x2*>(&Xt[buf][8 * fg + e][col]) = bf16x2{x0[e], x1[e]};
+        *reinterpret_cast<bf16x2*>(&Zt[buf][8 * fg + e][col]) = bf16x2{z0[e], z1[e]};
+        bsum[e] += (float)z0[e] + (float)z1[e];
+      }
+    }
+  };
+  auto operand = [&](const __bf16 (*tile)[kWP], int blk, int s) {
+    return *reinterpret_cast<const bf16x8*>(&tile[32 * blk + r][16 * (s ^ (blk & 3)) + 8 * qx]);
+  };
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < kWC / 16; ++s) {
+      const bf16x8 bz = operand(Zt[buf], wv, s);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) acc[i] = mfma_bf16(operand(Xt[buf], i, s), bz, acc[i]);
+      __builtin_amdgcn_sched_barrier(0);  // at most one sub-step of operands live
+    }
+  };
+
+  const int nchunks = (int)((rows + kWC - 1) / kWC);
+  const std::integral_constant<int, 0> set0;
+  const std::integral_constant<int, 1> set1;
+  load_chunk(set0, 0);
+  load_chunk(set1, 1);
+  store_chunk(set0, 0);
+  load_chunk(set0, 2);
+  __syncthreads();
+  for (int c = 0; c < nchunks; c += 2) {
+    compute(0);                // chunk c
+    store_chunk(set1, 1);      // chunk c+1 (zeros past the end; never read then)
+    load_chunk(set1, c + 3);
+    __syncthreads();
+    if (c + 1 >= nchunks) break;
+    compute(1);                // chunk c+1
+    store_chunk(set0, 0);      // chunk c+2
+    load_chunk(set0, c + 4);
+    __syncthreads();
+  }
+
+  // partial tile rows kk = 32 i + 8 (e / 4) + 4 q + e % 4, column nn, through a
+  // buffer resource over the tile: the VGPR offset is range-checked, so rows kk >= K
+  // are dropped
+  const int nn = 32 * wv + r;
+  const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(
+      J.part + (int64_t)split * J.K * J.N, 0, J.K * J.N * 4, 0x00020000);
+  const uint32_t vo = (uint32_t)((4 * q * J.N + nn) * 4);
+  if (nn < J.N) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        // copied out first: a bit_cast of a vector element subscript reads element 0
+        const float v = acc[i][e];
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), ro,
+                                              vo + (uint32_t)((32 * i + 8 * (e / 4) + (e % 4)) * J.N * 4), 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if (J.bpart) {
+    // lanes l and l + 32 hold the same feature group; then the 8 waves through LDS
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bsum[e] += shfl_xor(bsum[e], 32);
+    float* red = reinterpret_cast<float*>(&Xt[0][0][0]);  // [8 waves][32][8]
+    if (lane < NFG) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) red[(wv * NFG + lane) * 8 + e] = bsum[e];
+    }
+    __syncthreads();
+    if (wv == 0 && lane < NFG) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        float v = 0.f;
+#pragma unroll
+        for (int w = 0; w < 8; ++w) v += red[(w * NFG + lane) * 8 + e];
+        const int n = 8 * lane + e;
+        if (n < J.N) J.bpart[(int64_t)split * J.N + n] = v;
+      }
+    }
+  }
+}
+
 int wg16_blocks(int n) { return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8; }
 
 // All jobs of `a` share the tile shape (kb, nb) = wg16_blocks of their 32-feature blocks.
 int launch_wgrad_bf16(const Wgrad16Args& a, int kb, int nb, hipStream_t s) {
   const unsigned grid = (unsigned)(a.splits * a.njobs);
+  if (kb == 8 && nb == 8) {
+    wgrad_bf16_w8_kernel<<<grid, 512, 0, s>>>(a);
+    return 0;
+  }
 #define NERF_WG16(KB_, NB_) \
   if (kb == KB_ && nb == NB_) { wgrad_bf16_kernel<KB_, NB_><<<grid, 256, 0, s>>>(a); return 0; }
 #define NERF_WG16_ROW(KB_) NERF_WG16(KB_, 1) NERF_WG16(KB_, 2) NERF_WG16(KB_, 4) NERF_WG16(KB_, 8)
