@@ -20,8 +20,9 @@
 // rounding, which keeps the sign), two LDS stores and, in training, the stash
 // rows.  Each block's epilogue is issued right after its last MFMA, so it
 // overlaps the MFMAs of the next block.  Out-of-range samples of the last
-// workgroup compute the values of the last sample and write them to its rows
-// (identical bytes), so the epilogue has no per-lane branch.  One barrier per
+// workgroup compute the values of the last sample; their stash stores go
+// through a buffer resource over the workgroup's rows and fall outside its
+// range, so the epilogue has no per-lane branch.  One barrier per
 // layer; the sigma head and the rgb branch follow the reference's topology.
 #pragma once
 #include <type_traits>
@@ -202,10 +203,17 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
   const int nb0 = 2 * wv;  // this wave's output blocks of a 256-wide layer
   const uint32_t lane_off = 16u * (uint32_t)lane, g_off = 16u * (uint32_t)g;
   const __amdgpu_buffer_rsrc_t rs = packed_rsrc(pk, P.packed_bytes);
-  // stash row of sample block sb (clamped: out-of-range lanes rewrite the last row)
-  auto row = [&](int sb) {
-    const int64_t m = base + sb * 32 + j;
-    return stash + (m < M ? m : M - 1) * P.stash_per_sample;
+  // The stash rows of this workgroup as a buffer resource: 32-bit per-lane byte
+  // offsets (no 64-bit address per store), and rows past M are out of range, so
+  // their stores are dropped.  row_off(sb): this lane's row of sample block sb.
+  const int64_t wg_rows = M - base < kBlocks * 32 ? M - base : kBlocks * 32;
+  const __amdgpu_buffer_rsrc_t rst =
+      STASH ? __builtin_amdgcn_make_buffer_rsrc(stash + base * P.stash_per_sample, 0,
+                                                (int)(wg_rows * P.stash_per_sample * 2), 0x00020000)
+            : rs;
+  auto row_off = [&](int sb) { return (uint32_t)((sb * 32 + j) * P.stash_per_sample * 2); };
+  auto st16 = [&](bf16x8 v, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rst, off, 0, 0);
   };
   // first / second half of a layer's k-steps into wlo / whi
   auto load_half = [&](bf16x8 (&w)[kHalf][2], int m, bool enc, bool second) {
@@ -275,13 +283,14 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
       out[sb * kSlab + (2 * i + 1) * 64] = hi;
       if (STASH) {
         // as row chunks: features 32 (nb0 + i) + 16 h + 8 g .. + 7 (lo: h = 0, hi: h = 1)
-        __bf16* rr = row(sb) + st_off + 32 * (nb0 + i) + 8 * g;
-        *reinterpret_cast<bf16x8*>(rr) = swap_halves(lo);
-        *reinterpret_cast<bf16x8*>(rr + 16) = swap_halves(hi);
+        const uint32_t o = row_off(sb) + (uint32_t)((st_off + 32 * (nb0 + i) + 8 * g) * 2);
+        st16(swap_halves(lo), o);
+        st16(swap_halves(hi), o + 32);
         bits |= nz_bits(lo, hi) << (8 * i);
       }
     }
-    if constexpr (STASH && MASK) *reinterpret_cast<uint32_t*>(row(sb) + mask_off + 4 * wv + 2 * g) = bits;
+    if constexpr (STASH && MASK)
+      __builtin_amdgcn_raw_buffer_store_b32(bits, rst, row_off(sb) + (uint32_t)((mask_off + 4 * wv + 2 * g) * 2), 0, 0);
   };
 
   __syncthreads();  // encodings visible
@@ -362,14 +371,14 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) a[r] = (float)(__bf16)(a[r] > 0.f ? a[r] : 0.f);
       if (STASH) {
-        __bf16* rr = row(sb) + P.st_rbr + 32 * wv + 8 * g;
+        const uint32_t o = row_off(sb) + (uint32_t)((P.st_rbr + 32 * wv + 8 * g) * 2);
         const bf16x8 lo = acc_to_bf16(a, 0), hi = acc_to_bf16(a, 1);
-        *reinterpret_cast<bf16x8*>(rr) = swap_halves(lo);
-        *reinterpret_cast<bf16x8*>(rr + 16) = swap_halves(hi);
+        st16(swap_halves(lo), o);
+        st16(swap_halves(hi), o + 32);
         // ReLU bits of rgb-branch block wv: bits 0..7 and 16..23 of nz_bits as 16 bits
         const uint32_t b = nz_bits(lo, hi);
-        *reinterpret_cast<unsigned short*>(row(sb) + P.st_mask + 16 * D + 2 * wv + g) =
-            (unsigned short)((b & 0xFFu) | ((b >> 8) & 0xFF00u));
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)((b & 0xFFu) | ((b >> 8) & 0xFF00u)), rst,
+                                              row_off(sb) + (uint32_t)((P.st_mask + 16 * D + 2 * wv + g) * 2), 0, 0);
       }
       float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll
