@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--recompute-chunk", type=int, default=2048,
                     help="rays per recomputation chunk")
     ap.add_argument("--no-hbm-kernels", action="store_true",
-                    help="skip the timing of the HBM-bound sampling / compositing kernels")
+                    help="skip the timing of the HBM-bound sampling / compositing kernels and of "
+                         "the train step's MLP stages")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -125,6 +126,7 @@ def main():
         results[form] = measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches)
     head = results[forms[0]]
     hbm = None if args.no_hbm_kernels else measure_hbm_kernels(nerf, dev)
+    stages = None if args.no_hbm_kernels else measure_train_stages(nerf, dev)
     recompute = None
     if args.recompute_rays > 0:
         recompute = {p: measure_recompute(p, args, nerf, P, dev, batches) for p in ("fp32", "bf16")}
@@ -155,6 +157,7 @@ def main():
         "cpu_baseline": cpu,
         "forms": results,
         "hbm_kernels": hbm,
+        "train_stages": stages,
         "recompute": recompute,
     }
     if "learned_bf16" in results and "plain_fp32" in results and results["plain_fp32"]["train_rays_per_s"]:
@@ -294,6 +297,66 @@ def measure_hbm_kernels(nerf, dev, n=32768, reps=20):
         gbs = bpr * n / (us * 1e-6) / 1e9
         out[name] = {"avg_us": round(us, 2), "bytes_per_ray": bpr, "achieved_gbs": round(gbs, 1),
                      "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 3)}
+    return out
+
+
+def measure_train_stages(nerf, dev, rays=4096, spr=176, reps=20):
+    """The four MLP launches of the BF16 train step at the learned form's fine-model
+    shape (4096 rays x (48 + 128) samples = 720,896 samples), each launched alone
+    through the C ABI and timed with HIP events on its stream: the forward without
+    and with the activation stash, the dX chain and the weight gradients, with the
+    algorithmic FLOPs and HBM bytes of each.  Stash and dz rows are bf16 (bytes per
+    sample = 2 x the row length in elements)."""
+    import ctypes
+    import torch
+    L = nerf._lib
+    M = rays * spr
+    mlp = nerf.NerfMLP("fine", nerf.MLPConfig(), "bf16", seed=4, device=dev)
+    d = ctypes.byref(mlp._desc)
+    g = torch.Generator(device=dev).manual_seed(11)
+    xyz = torch.rand((M, 3), device=dev, generator=g) * 6 - 3
+    dirs = torch.nn.functional.normalize(torch.randn((M, 3), device=dev, generator=g), dim=1)
+    g_rgb = torch.randn((M, 3), device=dev, generator=g)
+    g_sig = torch.randn((M, 1), device=dev, generator=g)
+    rgb, sigma = torch.empty((M, 3), device=dev), torch.empty((M, 1), device=dev)
+    stash = mlp._alloc_stash(M)
+    dz = torch.empty((M, mlp.dz_per_sample), dtype=torch.bfloat16, device=dev)
+    grad = torch.empty((mlp.n_params,), device=dev)
+    ws = torch.empty((int(L.load().nerf_mlp_wgrad_workspace_bytes(d, M)),), dtype=torch.uint8, device=dev)
+    st_b, dz_b = 2 * mlp.stash_per_sample, 2 * mlp.dz_per_sample
+    stages = {
+        "fwd": (MLP_FLOP_PER_SAMPLE, 0, lambda: mlp._launch_forward(
+            M, xyz, dirs, None, None, None, rgb, sigma, None)),
+        "fwd_stash": (MLP_FLOP_PER_SAMPLE, st_b, lambda: mlp._launch_forward(
+            M, xyz, dirs, None, None, None, rgb, sigma, stash)),
+        # dX of every layer but the input sides of dense_0, dense_5 (63 x 256 each) and dense_9 (27 x 128)
+        "bwd": (2 * (593408 - 35712), dz_b, lambda: L.call(
+            "nerf_mlp_backward", d, mlp.prec, L.ptr(mlp.packed), M, L.ptr(g_rgb), L.ptr(g_sig), L.ptr(rgb),
+            L.ptr(sigma), L.ptr(stash), L.ptr(dz), L.stream(dev))),
+        "wgrad": (MLP_FLOP_PER_SAMPLE, st_b + dz_b, lambda: L.call(
+            "nerf_mlp_weight_grads", d, mlp.prec, M, L.ptr(stash), L.ptr(dz), L.ptr(grad), 0, L.ptr(ws),
+            L.stream(dev))),
+    }
+    out = {"samples": M, "workload": f"fine model of the learned BF16 train step: {rays} rays x {spr} samples"}
+    for name, (flop, nbytes, fn) in stages.items():
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        r = {"avg_us": round(us, 1), "flop_per_sample": flop,
+             "tflops": round(flop * M / (us * 1e-6) / 1e12, 1)}
+        if nbytes:
+            gbs = nbytes * M / (us * 1e-6) / 1e9
+            r.update({"hbm_bytes_per_sample": nbytes, "achieved_gbs": round(gbs, 1),
+                      "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 3)})
+        out[name] = r
+    del stash, dz, ws
+    torch.cuda.empty_cache()
     return out
 
 

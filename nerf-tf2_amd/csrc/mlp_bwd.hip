@@ -539,11 +539,13 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   }
 }
 
-// The 256 x 256 tile with 8 waves (two per SIMD): wave w owns output column block
-// w for all 8 row blocks (128 accumulators), which leaves registers for two
-// chunks of loads in flight per thread (chunk k is loaded into register set
-// k % 2 two iterations before it goes to LDS).  Same chunking, transposed
-// XOR-swizzled LDS tiles and fixed-order reductions as wgrad_bf16_kernel.
+// The 256 x 256 tile with 8 waves (two per SIMD): wave w owns the 2 x 4 blocks
+// of output column blocks 2 (w % 4) + {0, 1} and row blocks 4 (w / 4) + {0..3}
+// (128 accumulators), so a k-step reads 6 operands from LDS for 8 MFMAs, and
+// registers remain for two chunks of loads in flight per thread (chunk k is
+// loaded into register set k % 2 two iterations before it goes to LDS).  Same
+// chunking, transposed XOR-swizzled LDS tiles and fixed-order reductions as
+// wgrad_bf16_kernel.
 __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
   constexpr int NFG = 32, R = 2, PP = 512 / NFG;  // feature groups, rounds, sample pairs per round
   typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -606,12 +608,17 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
   auto operand = [&](const __bf16 (*tile)[kWP], int blk, int s) {
     return *reinterpret_cast<const bf16x8*>(&tile[32 * blk + r][16 * (s ^ (blk & 3)) + 8 * qx]);
   };
+  const int cb0 = 2 * (wv & 3), rb0 = 4 * (wv >> 2);  // acc[2 i + c]: row block rb0 + i, column block cb0 + c
   auto compute = [&](int buf) {
 #pragma unroll
     for (int s = 0; s < kWC / 16; ++s) {
-      const bf16x8 bz = operand(Zt[buf], wv, s);
+      const bf16x8 bz0 = operand(Zt[buf], cb0, s), bz1 = operand(Zt[buf], cb0 + 1, s);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) acc[i] = mfma_bf16(operand(Xt[buf], i, s), bz, acc[i]);
+      for (int i = 0; i < 4; ++i) {
+        const bf16x8 ax = operand(Xt[buf], rb0 + i, s);
+        acc[2 * i] = mfma_bf16(ax, bz0, acc[2 * i]);
+        acc[2 * i + 1] = mfma_bf16(ax, bz1, acc[2 * i + 1]);
+      }
       __builtin_amdgcn_sched_barrier(0);  // at most one sub-step of operands live
     }
   };
@@ -636,24 +643,28 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
     __syncthreads();
   }
 
-  // partial tile rows kk = 32 i + 8 (e / 4) + 4 q + e % 4, column nn, through a
-  // buffer resource over the tile: the VGPR offset is range-checked, so rows kk >= K
-  // are dropped
-  const int nn = 32 * wv + r;
+  // partial tile rows kk = 32 (rb0 + i) + 8 (e / 4) + 4 q + e % 4, column
+  // nn = 32 (cb0 + c) + r, through a buffer resource over the tile: the VGPR offset
+  // is range-checked, so rows kk >= K are dropped
   const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(
       J.part + (int64_t)split * J.K * J.N, 0, J.K * J.N * 4, 0x00020000);
-  const uint32_t vo = (uint32_t)((4 * q * J.N + nn) * 4);
-  if (nn < J.N) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+  for (int c = 0; c < 2; ++c) {
+    const int nn = 32 * (cb0 + c) + r;
+    const uint32_t vo = (uint32_t)((4 * q * J.N + nn) * 4);
+    if (nn < J.N) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        // copied out first: a bit_cast of a vector element subscript reads element 0
-        const float v = acc[i][e];
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), ro,
-                                              vo + (uint32_t)((32 * i + 8 * (e / 4) + (e % 4)) * J.N * 4), 0, 0);
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          // copied out first: a bit_cast of a vector element subscript reads element 0
+          const float v = acc[2 * i + c][e];
+          __builtin_amdgcn_raw_buffer_store_b32(
+              __builtin_bit_cast(uint32_t, v), ro,
+              vo + (uint32_t)((32 * (rb0 + i) + 8 * (e / 4) + (e % 4)) * J.N * 4), 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
     }
   }
   if (J.bpart) {
