@@ -702,7 +702,8 @@ int launch_wgrad_bf16(const Wgrad16Args& a, int kb, int nb, hipStream_t s) {
 #define NERF_WG16(KB_, NB_) \
   if (kb == KB_ && nb == NB_) { wgrad_bf16_kernel<KB_, NB_><<<grid, 256, 0, s>>>(a); return 0; }
 #define NERF_WG16_ROW(KB_) NERF_WG16(KB_, 1) NERF_WG16(KB_, 2) NERF_WG16(KB_, 4) NERF_WG16(KB_, 8)
-  NERF_WG16_ROW(1) NERF_WG16_ROW(2) NERF_WG16_ROW(4) NERF_WG16_ROW(8)
+  NERF_WG16_ROW(1) NERF_WG16_ROW(2) NERF_WG16_ROW(4)
+  NERF_WG16(8, 1) NERF_WG16(8, 2) NERF_WG16(8, 4)  // (8, 8): wgrad_bf16_w8_kernel above
 #undef NERF_WG16_ROW
 #undef NERF_WG16
   return -1;
