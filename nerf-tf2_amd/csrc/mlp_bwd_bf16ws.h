@@ -137,13 +137,12 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   }
   __syncthreads();
 
-  // Epilogue of sample block sb: ReLU mask (MASKED) from the stash activations at
-  // feature offset st_off, bf16 into LDS (B layout; out may be null) and the dz row
+  // Epilogue of sample block sb: ReLU mask (MASKED) from the forward's bit word
+  // `bits` of this block, bf16 into LDS (B layout; out may be null) and the dz row
   // at dz_off.
-  auto epilogue = [&](auto masked_c, auto lds_c, lds_b8* out, int64_t mask_off, int64_t dz_off, int sb) {
+  auto epilogue = [&](auto masked_c, auto lds_c, lds_b8* out, uint32_t bits, int64_t dz_off, int sb) {
     constexpr bool MASKED = decltype(masked_c)::value, TO_LDS = decltype(lds_c)::value;
     const uint32_t zr = dz_row(sb);
-    const uint32_t bits = MASKED ? ld_u32(rst, mask_row(sb), (mask_off + 4 * wv) * 2) : 0u;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int64_t f0 = 32 * (nb0 + i);
@@ -171,7 +170,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
     run_steps<0, KR / 2, false>(acc, wlo, cb_unused, in_l, in_l, none);
     load_w<kT / 2, 0>(wlo, rs, P.p_bwd[D], C::NB, nb0, lane_off);
     run_steps<KR / 2, KR / 2, false>(acc, whi, cb_unused, in_l, in_l,
-                                     [&](int sb) { epilogue(std::false_type{}, std::true_type{}, out_l, 0, P.dz_bott, sb); });
+                                     [&](int sb) { epilogue(std::false_type{}, std::true_type{}, out_l, 0u, P.dz_bott, sb); });
     load_w<kT / 2, kT / 2>(whi, rs, P.p_bwd[D], C::NB, nb0, lane_off);
     __syncthreads();
   }
@@ -183,6 +182,14 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
     constexpr int cur = (s + 1) & 1;      // buffer holding this step's input dz
     lds_b8* const in_l = lds_lane(s_act[cur], 0, lane);
     lds_b8* const out_l = lds_lane(s_act[cur ^ 1], 2 * nb0 * 64, lane);  // unused for layer 1
+    // the forward's ReLU bit words of this step's output for the four sample blocks,
+    // requested before the step's weight loads and dz stores: vmcnt retires vector
+    // memory operations in issue order, so a load issued behind them would also wait
+    // for them
+    uint32_t mbits[kBlocks];
+#pragma unroll
+    for (int sb = 0; sb < kBlocks; ++sb)
+      mbits[sb] = ld_u32(rst, mask_row(sb), (P.st_mask + 16 * (layer - 1) + 4 * wv) * 2);
     if constexpr (s == 0) {
       // accumulators start from the sigma-head term W_sigma dz_sigma
       const uint32_t hs_off = 4u * (uint32_t)(g * C::KH);
@@ -216,7 +223,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
     run_steps<H, H, false>(acc, whi, cb_unused, in_l, in_l,
                            [&](int sb) {
                              epilogue(std::true_type{}, std::integral_constant<bool, (layer > 1)>{}, out_l,
-                                      P.st_mask + 16 * (layer - 1), P.dz_h[layer - 1], sb);
+                                      mbits[sb], P.dz_h[layer - 1], sb);
                            });
     if constexpr (layer > 1) {
       load_w<H, H>(whi, rs, P.p_bwd[layer - 1], C::NB, nb0, lane_off);
