@@ -70,7 +70,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU; "
-              "the path shards by ray ranges (see DESIGN.md) but is not run on several GPUs.",
+              "the path shards by ray ranges but is not run on several GPUs.",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
