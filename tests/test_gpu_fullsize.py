@@ -84,3 +84,38 @@ def test_learned_bf16_chunk_invariants(nerf):
     assert float(missed.min()) >= 0.0 and float(missed.max()) <= 1.0 + 1e-5
     assert 0.0 <= pn.last_fallback_fraction <= 1.0
     assert bool(torch.isfinite(pp_f["pred_depth"]).all())
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_config3_recompute_matches_stash(nerf, prec):
+    """Config 3's batch, 16,384 rays x (64 + 128) samples: the train-step gradients
+    with activation recomputation in 2,048-ray chunks equal those with every
+    activation stashed (only the split-K summation order differs), and the step's
+    peak HBM falls below a fifth."""
+    n = 16384
+    ro, rd, near, far = _rays_batch(nerf, n, seed=47, H=800, W=800, focal=1111.1)
+    gen = torch.Generator(device=DEV).manual_seed(9)
+    rgb = torch.rand((n, 3), device=DEV, generator=gen)
+    u_c = torch.rand((n, 64), device=DEV, generator=gen)
+    u_f = torch.rand((n, 128), device=DEV, generator=gen)
+    grads, peaks = [], []
+    for chunk in (None, 2048):
+        net = nerf.NeRF(nerf.Params(), prec, seed=6)
+        for mdl in (net.coarse_model, net.fine_model):
+            mdl.recompute_chunk = chunk
+            mdl.params.requires_grad_(True)
+            mdl.params.grad = None
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        pc, pf = net.forward(ro, rd, near, far, u_c, u_f)
+        (nerf.ray_utils.mse_loss(rgb, pc["pred_rgb"]) + nerf.ray_utils.mse_loss(rgb, pf["pred_rgb"])).backward()
+        torch.cuda.synchronize()
+        peaks.append(torch.cuda.max_memory_allocated() - base)
+        grads.append([net.coarse_model.params.grad.clone(), net.fine_model.params.grad.clone()])
+        del net, pc, pf
+        torch.cuda.empty_cache()
+    for a, b in zip(*grads):
+        assert bool(torch.isfinite(a).all())
+        assert torch.allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.abs().max()))
+    assert peaks[1] < 0.2 * peaks[0], peaks
