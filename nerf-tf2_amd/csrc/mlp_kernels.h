@@ -52,21 +52,34 @@ __device__ __forceinline__ void load_sample(int64_t m, const float* __restrict__
                                             const float* __restrict__ rd,
                                             const float* __restrict__ tv, int spr, float (&x)[3],
                                             float (&d)[3]) {
+  // scalars first, then the arrays: per-component stores under the branches kept the
+  // arrays in scratch memory
+  float x0, x1, x2, d0 = 0.f, d1 = 0.f, d2 = 0.f;
   if (ro) {
     const int64_t r = m / spr;
     const float t = tv[m];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      d[c] = rd[3 * r + c];
-      x[c] = fadd(ro[3 * r + c], fmul(t, d[c]));
-    }
+    d0 = rd[3 * r];
+    d1 = rd[3 * r + 1];
+    d2 = rd[3 * r + 2];
+    x0 = fadd(ro[3 * r], fmul(t, d0));
+    x1 = fadd(ro[3 * r + 1], fmul(t, d1));
+    x2 = fadd(ro[3 * r + 2], fmul(t, d2));
   } else {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      x[c] = xyz[3 * m + c];
-      d[c] = dirs ? dirs[3 * m + c] : 0.f;
+    x0 = xyz[3 * m];
+    x1 = xyz[3 * m + 1];
+    x2 = xyz[3 * m + 2];
+    if (dirs) {
+      d0 = dirs[3 * m];
+      d1 = dirs[3 * m + 1];
+      d2 = dirs[3 * m + 2];
     }
   }
+  x[0] = x0;
+  x[1] = x1;
+  x[2] = x2;
+  d[0] = d0;
+  d[1] = d1;
+  d[2] = d2;
 }
 
 // Positional encoding (model.py:309-330) directly in the lane-half slots of
@@ -76,6 +89,11 @@ template <int L, int K>
 __device__ __forceinline__ void encode_slots(const float (&x)[3], int g, float (&out)[K]) {
   constexpr int P0 = (3 * L - 2) / 2;
   constexpr float kPi = 3.14159265358979323846f;
+  // the components as register values: a select between elements of the array by a
+  // lane-dependent index would otherwise be folded into an indexed load, which keeps
+  // the array in scratch memory
+  float x0 = x[0], x1 = x[1], x2 = x[2];
+  asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2));
 #pragma unroll
   for (int k = 0; k < K / 2; ++k) {
     const int p = g ? P0 + k : k - 2;
@@ -83,14 +101,14 @@ __device__ __forceinline__ void encode_slots(const float (&x)[3], int g, float (
     float s = 0.f, c = 0.f;
     if (is_pair) {
       const int comp = p / L, l = p - comp * L;
-      const float xc = comp == 0 ? x[0] : (comp == 1 ? x[1] : x[2]);
+      const float xc = comp == 0 ? x0 : (comp == 1 ? x1 : x2);
       const float arg = fmul(xc, ldexpf(kPi, l));
       sincosf(arg, &s, &c);
     }
     float v0 = s, v1 = c;
     if (!is_pair) {
-      v0 = (g == 0 && k == 0) ? x[0] : ((g == 0 && k == 1) ? x[2] : 0.f);
-      v1 = (g == 0 && k == 0) ? x[1] : 0.f;
+      v0 = (g == 0 && k == 0) ? x0 : ((g == 0 && k == 1) ? x2 : 0.f);
+      v1 = (g == 0 && k == 0) ? x1 : 0.f;
     }
     out[2 * k] = v0;
     out[2 * k + 1] = v1;
