@@ -36,7 +36,7 @@ RENDER_FLOP_PER_RAY = (64 + 192) * MLP_FLOP_PER_SAMPLE
 # same counter on a 1 GiB device copy), from the rocprofv3 --pmc runs whose commands
 # and figures are in profiles/mlp_fwd_f32_render800.md and
 # profiles/mlp_fwd_bf16ws_render800.md.
-TRAFFIC_BYTES_PER_LAUNCH = {"fp32": 454.6e6, "bf16": 183.3e6}
+TRAFFIC_BYTES_PER_LAUNCH = {"fp32": 152.6e6, "bf16": 142.9e6}
 
 
 def parse():
