@@ -5,9 +5,11 @@ a hyphen).  Modules:
   ray_utils  reference-named sampling / compositing functions (HIP kernels)
   model      NerfMLP (fused encoder + MLP), NeRF, AdamTF
   scene      the procedural Gaussian-blob dataset (float64 ground truth)
+  pose_utils the reference's scene preprocessing (new world frame, scene scale,
+             image / intrinsics scaling), host-side float64
   _lib       ctypes binding of libnerf_hip.so (include/nerf_hip.h)
 """
-from . import _lib, ray_utils, model, scene, graphs, render, checkpoint  # noqa: F401
+from . import _lib, ray_utils, model, scene, graphs, render, checkpoint, pose_utils  # noqa: F401
 from .model import MLPConfig, NeRF, NerfMLP, AdamTF, DynamicLossScaler, get_coarse_or_fine_model  # noqa: F401
 from .proposal import ProposalNeRF  # noqa: F401
 from .graphs import GraphedTrainStep  # noqa: F401
