@@ -275,7 +275,10 @@ def measure_hbm_kernels(nerf, dev, n=32768, reps=20):
             ptr(edges), ptr(widths), ptr(t_c), None, None, st)),
         "fine128_kernel (Nc=64, Nf=128)": (4 * (3 * Nc + (Nc + 1) + Nf + S), lambda: call(
             "nerf_create_input_batch_fine", n, Nc, Nf, ptr(ro), ptr(rd), ptr(w_c), ptr(edges), ptr(widths),
-            ptr(t_c), ptr(u_f), ptr(t), None, None, None, None, st)),
+            ptr(t_c), ptr(u_f), ptr(t), None, None, None, None, 0, None, st)),
+        "fine128_kernel compensated (Nc=64, Nf=128)": (4 * (3 * Nc + (Nc + 1) + Nf + S), lambda: call(
+            "nerf_create_input_batch_fine", n, Nc, Nf, ptr(ro), ptr(rd), ptr(w_c), ptr(edges), ptr(widths),
+            ptr(t_c), ptr(u_f), ptr(t), None, None, None, None, 1, None, st)),
         "composite_fwd_kernel (S=192)": (24 * S + 20, lambda: call(
             "nerf_composite_forward", n, S, 1, ptr(sigma), ptr(rgb), ptr(t), ptr(w_out), ptr(rgb_out), ptr(depth),
             ptr(acc), st)),
@@ -335,7 +338,7 @@ def measure_train_stages(nerf, dev, rays=4096, spr=176, reps=20):
             L.ptr(sigma), L.ptr(stash), L.ptr(dz), L.stream(dev))),
         "wgrad": (MLP_FLOP_PER_SAMPLE, st_b + dz_b, lambda: L.call(
             "nerf_mlp_weight_grads", d, mlp.prec, M, L.ptr(stash), L.ptr(dz), L.ptr(grad), 0, L.ptr(ws),
-            L.stream(dev))),
+            ws.numel(), L.stream(dev))),
     }
     out = {"samples": M, "workload": f"fine model of the learned BF16 train step: {rays} rays x {spr} samples"}
     for name, (flop, nbytes, fn) in stages.items():

@@ -93,16 +93,25 @@ int32_t nerf_create_input_batch_coarse(int64_t n_rays, int32_t n_coarse, int32_t
  *   u_vals (N, n_fine).  Outputs: t_vals (N, n_coarse+n_fine) sorted;
  *   optional (may be NULL): piece_idxs int32 (N, n_fine) (ray_utils.py:363),
  *   t_vals_fine (N, n_fine) in draw order (ray_utils.py:376),
- *   xyz_inputs / dir_inputs (N*(n_coarse+n_fine), 3).
+ *   xyz_inputs / dir_inputs (N*(n_coarse+n_fine), 3),
+ *   cdf (N, n_coarse+1): the cdf the draws are inverted against
+ *   (ray_utils.py:342-345, leading 0 included).
+ *   cdf_mode: NERF_CDF_TF_ORDER computes the denominator and the cdf in the
+ *   reference's float32 operation order (bin indices bit-exact with it);
+ *   NERF_CDF_COMPENSATED (the BF16 and learned-proposal forms) computes them
+ *   with compensated summation: each cdf value within about 1 ulp of the exact
+ *   prefix ratio.
  *   n_coarse <= 64, n_fine <= 256.
  * ---------------------------------------------------------------------- */
+#define NERF_CDF_TF_ORDER 0
+#define NERF_CDF_COMPENSATED 1
 int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse, int32_t n_fine,
                                      const float* rays_o, const float* rays_d,
                                      const float* bin_weights, const float* bin_edges,
                                      const float* bin_widths, const float* t_vals_coarse,
                                      const float* u_vals, float* t_vals, int32_t* piece_idxs,
                                      float* t_vals_fine, float* xyz_inputs, float* dir_inputs,
-                                     void* stream);
+                                     int32_t cdf_mode, float* cdf, void* stream);
 
 /* ------------------------------------------------------------------------
  * post_process_model_output (nerf/utils/ray_utils.py:484-551) with
@@ -190,7 +199,10 @@ int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precision, const vo
 
 /* Weight gradients: grad_flat (param_count floats, flat layout) = or +=
  * (accumulate != 0) sum over the M samples of X^T dz for every layer, and the
- * bias sums.  workspace: nerf_mlp_wgrad_workspace_bytes(desc, M) bytes.
+ * bias sums.  workspace: a device buffer of workspace_bytes bytes, at least
+ * nerf_mlp_wgrad_workspace_bytes(desc, M) (a bound for both precisions); a
+ * smaller buffer than the call's launches use is rejected with
+ * NERF_ERR_INVALID_ARG before anything is launched.
  * The summation order is fixed (deterministic, no float atomics).
  * M == 0 (an empty batch): grad_flat is zeroed (or left as is when
  * accumulating); stash, dz and workspace may then be NULL.  Every other entry
@@ -199,7 +211,8 @@ int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precision, const vo
 int64_t nerf_mlp_wgrad_workspace_bytes(const nerf_mlp_desc* desc, int64_t M);
 int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
                               const void* stash, const void* dz, float* grad_flat,
-                              int32_t accumulate, void* workspace, void* stream);
+                              int32_t accumulate, void* workspace, int64_t workspace_bytes,
+                              void* stream);
 
 /* ------------------------------------------------------------------------
  * Losses and the optimizer step (model.py:32, 148-171, 413-418).

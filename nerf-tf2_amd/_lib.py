@@ -42,7 +42,7 @@ _SIGS = {
     "nerf_get_rays_tf": (_i32, [_i32, _i32, _fp, _fp, _vp, _vp, _vp]),
     "nerf_sample_batch": (_i32, [_i64, _i32, _i32, _i32] + [_vp] * 11 + [_vp]),
     "nerf_create_input_batch_coarse": (_i32, [_i64, _i32, _i32, _i32] + [_vp] * 10 + [_vp]),
-    "nerf_create_input_batch_fine": (_i32, [_i64, _i32, _i32] + [_vp] * 12 + [_vp]),
+    "nerf_create_input_batch_fine": (_i32, [_i64, _i32, _i32] + [_vp] * 12 + [_i32, _vp, _vp]),
     "nerf_composite_forward": (_i32, [_i64, _i32, _i32] + [_vp] * 7 + [_vp]),
     "nerf_composite_forward_f64": (_i32, [_i64, _i32, _i32] + [_vp] * 7 + [_vp]),
     "nerf_composite_backward": (_i32, [_i64, _i32, _i32] + [_vp] * 9 + [_vp]),
@@ -56,7 +56,7 @@ _SIGS = {
                                 _vp, _vp]),
     "nerf_mlp_backward": (_i32, [_descp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nerf_mlp_wgrad_workspace_bytes": (_i64, [_descp, _i64]),
-    "nerf_mlp_weight_grads": (_i32, [_descp, _i32, _i64, _vp, _vp, _vp, _i32, _vp, _vp]),
+    "nerf_mlp_weight_grads": (_i32, [_descp, _i32, _i64, _vp, _vp, _vp, _i32, _vp, _i64, _vp]),
     "nerf_mse_grad": (_i32, [_i64, _vp, _vp, _f32, _vp, _vp, _vp]),
     "nerf_adam_tf": (_i32, [_i64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _vp, _vp, _vp]),
     "nerf_check_finite": (_i32, [_i64, _vp, _vp, _vp]),

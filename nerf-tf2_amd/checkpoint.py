@@ -30,18 +30,16 @@ import numpy as np
 import torch
 
 from .model import MLPConfig, NerfMLP
+from .model import keras_variables as model_keras_variables
 
 ADAM_BETA_1, ADAM_BETA_2, ADAM_DECAY = 0.9, 0.999, 0.0
 
 
 def keras_variables(config: MLPConfig, model_name: str) -> List[Tuple[str, Tuple[int, ...]]]:
     """[(variable name without ':0', shape)] of one model in Keras trainable order
-    (model.py:355-387: dense_0..7, sigma, dense_8, dense_9, rgb; kernel then bias)."""
-    out = []
-    for name, fi, fo in config.layer_shapes():
-        out.append((f"{model_name}/{name}/kernel", (fi, fo)))
-        out.append((f"{model_name}/{name}/bias", (fo,)))
-    return out
+    (model.py:355-392: dense_0 .. dense_9, rgb, sigma; kernel then bias; the order
+    is derived in MLPConfig.layer_shapes)."""
+    return [(n[:-2], shape) for n, shape in model_keras_variables(config, model_name)]
 
 
 def _split(mlp: NerfMLP, flat: torch.Tensor) -> List[np.ndarray]:
@@ -66,12 +64,10 @@ def save_weights_npz(path: str, mlp: NerfMLP) -> None:
 
 
 def load_weights_npz(path: str, mlp: NerfMLP) -> None:
+    """Reads every variable by name (whatever order the file's `names` holds) and
+    checks its shape (NerfMLP.set_weights)."""
     with np.load(path, allow_pickle=False) as data:
-        arrays = [data[f"{name}:0"] for name, _ in keras_variables(mlp.config, mlp.name)]
-    for (name, shape), a in zip(keras_variables(mlp.config, mlp.name), arrays):
-        if tuple(a.shape) != shape:
-            raise ValueError(f"{name}: saved shape {a.shape}, expected {shape}")
-    mlp.set_weights(arrays)
+        mlp.set_weights({k: data[k] for k in data.files if k != "names"})
 
 
 def load_weights_h5(path: str, mlp: NerfMLP) -> None:

@@ -813,12 +813,12 @@ int wgrad_jobs(const MlpPlan& P, WgradJob* jobs) {
     add(P.st_h[i - 1], W, P.dz_h[i], W, W, 0, 0, 0, i, 1);
     if (i - 1 == P.skip) add(P.st_encx, 2 * P.KX, P.dz_h[i], W, W, P.LX, P.KX, W, i, 0);
   }
-  add(P.st_h[P.D - 1], W, P.dz_sigma, 1, 1, 0, 0, 0, P.D, 1);  // sigma
+  add(P.st_h[P.D - 1], W, P.dz_sigma, 1, 1, 0, 0, 0, P.kl_sigma, 1);
   if (P.rgb) {
-    add(P.st_h[P.D - 1], W, P.dz_bott, W, W, 0, 0, 0, P.D + 1, 1);
-    add(P.st_bott, W, P.dz_rbr, 32 * P.NBR, P.WR, 0, 0, 0, P.D + 2, 1);
-    add(P.st_encd, 2 * P.KD, P.dz_rbr, 32 * P.NBR, P.WR, P.LD, P.KD, W, P.D + 2, 0);
-    add(P.st_rbr, P.WR, P.dz_rgb, 3, 3, 0, 0, 0, P.D + 3, 1);
+    add(P.st_h[P.D - 1], W, P.dz_bott, W, W, 0, 0, 0, P.kl_bott, 1);
+    add(P.st_bott, W, P.dz_rbr, 32 * P.NBR, P.WR, 0, 0, 0, P.kl_rbr, 1);
+    add(P.st_encd, 2 * P.KD, P.dz_rbr, 32 * P.NBR, P.WR, P.LD, P.KD, W, P.kl_rbr, 0);
+    add(P.st_rbr, P.WR, P.dz_rgb, 3, 3, 0, 0, 0, P.kl_rgb, 1);
   }
   return n;
 }
@@ -923,6 +923,8 @@ extern "C" int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precisio
 // widest layer (kmax x W) and their bias rows, s = the larger of the float32
 // split count and the bf16 bound: a group of n <= kMaxJobs same-shape jobs uses
 // n x wg16_splits(M, n) <= min(256 + kMaxJobs - 1, kMaxJobs x ceil(M / 256)) tiles.
+// This is the size callers are told to allocate; wgrad_used_floats is what one
+// call's launches actually touch (never more).
 static int64_t wgrad_workspace_floats(const MlpPlan& P, int64_t M) {
   int64_t b16 = kMaxJobs * ((M + 255) / 256);
   if (b16 > 256 + kMaxJobs - 1) b16 = 256 + kMaxJobs - 1;
@@ -934,33 +936,81 @@ static int64_t wgrad_workspace_floats(const MlpPlan& P, int64_t M) {
   return s * (kmax * P.W + P.W);
 }
 
+// The bf16 launches: jobs grouped by tile shape (up to kMaxJobs per launch),
+// each group with sg splits per job; a group's partial tiles start at the
+// workspace base (groups run one after another on the stream).
+struct Wg16Group {
+  int members[kMaxJobs];
+  int n, kb, nb;
+  int64_t sg;
+};
+
+static int wg16_groups(const MlpPlan& P, int64_t M, const WgradJob* jobs, int nj, Wg16Group* groups) {
+  bool done[2 * NERF_MAX_LAYERS] = {};
+  const int64_t row_bytes = 2 * (P.stash_per_sample > P.dz_per_sample ? P.stash_per_sample : P.dz_per_sample);
+  int ng = 0;
+  for (int t0 = 0; t0 < nj; ++t0) {
+    if (done[t0]) continue;
+    if (jobs[t0].K > 256 || jobs[t0].N > 256) return -1;
+    Wg16Group& g = groups[ng++];
+    g.kb = wg16_blocks((jobs[t0].K + 31) / 32);
+    g.nb = wg16_blocks((jobs[t0].N + 31) / 32);
+    g.n = 0;
+    for (int t = t0; t < nj && g.n < kMaxJobs; ++t)
+      if (!done[t] && wg16_blocks((jobs[t].K + 31) / 32) == g.kb && wg16_blocks((jobs[t].N + 31) / 32) == g.nb) {
+        g.members[g.n++] = t;
+        done[t] = true;
+      }
+    g.sg = wg16_splits(M, g.n);
+    if (g.sg < wg16_min_splits(M, row_bytes)) g.sg = wg16_min_splits(M, row_bytes);
+  }
+  return ng;
+}
+
+static int64_t wg16_group_floats(const Wg16Group& g, const WgradJob* jobs) {
+  int64_t f = 0;
+  for (int i = 0; i < g.n; ++i) {
+    const WgradJob& J = jobs[g.members[i]];
+    f += g.sg * J.K * J.N + (J.bias ? g.sg * J.N : 0);
+  }
+  return f;
+}
+
+// Workspace floats the launches of one call use (M > 0), or -1 for a layer the
+// bf16 GEMM cannot take.
+static int64_t wgrad_used_floats(const MlpPlan& P, int32_t precision, int64_t M) {
+  WgradJob jobs[2 * NERF_MAX_LAYERS];
+  const int nj = wgrad_jobs(P, jobs);
+  if (precision == NERF_PREC_BF16) {
+    Wg16Group groups[2 * NERF_MAX_LAYERS];
+    const int ng = wg16_groups(P, M, jobs, nj, groups);
+    if (ng < 0) return -1;
+    int64_t need = 0;
+    for (int i = 0; i < ng; ++i) {
+      const int64_t f = wg16_group_floats(groups[i], jobs);
+      if (f > need) need = f;
+    }
+    return need;
+  }
+  const int64_t kmax = (P.W > 2 * P.KX ? P.W : 2 * P.KX);
+  return splits_for(M) * (kmax * P.W + P.W);
+}
+
 extern "C" int64_t nerf_mlp_wgrad_workspace_bytes(const nerf_mlp_desc* desc, int64_t M) {
   MlpPlan P;
   if (nerf_plan_from_desc(desc, 0, &P) || M < 0) return -1;
   return wgrad_workspace_floats(P, M) * 4 + 256;
 }
 
-extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
-                                         const void* stash, const void* dz, float* grad_flat,
-                                         int32_t accumulate, void* workspace, void* stream) {
-  MlpPlan P;
-  NERF_CHECK_ARG(nerf_plan_from_desc(desc, precision, &P) == 0,
-                 "nerf_mlp_weight_grads: unsupported MLP description or precision");
-  NERF_CHECK_ARG(M >= 0 && grad_flat, "nerf_mlp_weight_grads: bad arguments");
-  hipStream_t s = as_stream(stream);
-  if (M == 0) {  // empty batch: the gradient is zero (a fill kernel, no memset node in a graph)
-    if (!accumulate)
-      zero_kernel<<<(unsigned)((P.n_params + 255) / 256 < 4096 ? (P.n_params + 255) / 256 : 4096), 256, 0, s>>>(
-          grad_flat, P.n_params);
-    NERF_LAUNCH_CHECK("nerf_mlp_weight_grads");
-    return NERF_OK;
-  }
-  NERF_CHECK_ARG(stash && dz && workspace, "nerf_mlp_weight_grads: bad arguments");
+// The launches of one weight-gradient call (M > 0) into the partial-tile buffer
+// `part` of at least wgrad_used_floats(P, precision, M) floats.
+static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const void* stash,
+                         const void* dz, float* grad_flat, int32_t accumulate, float* part,
+                         hipStream_t s) {
   WgradJob jobs[2 * NERF_MAX_LAYERS];
   const int nj = wgrad_jobs(P, jobs);
   const int64_t splits = splits_for(M);
   const int64_t mps = (M + splits - 1) / splits;
-  float* part = reinterpret_cast<float*>(workspace);
   auto reduce_args = [&](const WgradJob& J, const float* jp, const float* jb, int nsplit) {
     ReduceArgs r;
     r.part = jp;
@@ -984,25 +1034,14 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
     const int64_t rb = (tot + 255) / 256;
     return (int)(rb > 1024 ? 1024 : rb);
   };
-  auto reduce = [&](const WgradJob& J, const float* jp, const float* jb, int nsplit) {
-    wgrad_reduce_kernel<<<(unsigned)reduce_blocks(J), 256, 0, s>>>(reduce_args(J, jp, jb, nsplit));
-  };
   if (precision == NERF_PREC_BF16) {
-    // group the jobs by tile shape; each group is one launch with splits / njobs splits per job
-    bool done[2 * NERF_MAX_LAYERS] = {};
-    for (int t0 = 0; t0 < nj; ++t0) {
-      if (done[t0]) continue;
-      const int kb = wg16_blocks((jobs[t0].K + 31) / 32), nb = wg16_blocks((jobs[t0].N + 31) / 32);
-      NERF_CHECK_ARG(jobs[t0].K <= 256 && jobs[t0].N <= 256, "nerf_mlp_weight_grads: layer too wide");
-      int members[kMaxJobs], n = 0;
-      for (int t = t0; t < nj && n < kMaxJobs; ++t)
-        if (!done[t] && wg16_blocks((jobs[t].K + 31) / 32) == kb && wg16_blocks((jobs[t].N + 31) / 32) == nb) {
-          members[n++] = t;
-          done[t] = true;
-        }
-      const int64_t row_bytes = 2 * (P.stash_per_sample > P.dz_per_sample ? P.stash_per_sample : P.dz_per_sample);
-      int64_t sg = wg16_splits(M, n);
-      if (sg < wg16_min_splits(M, row_bytes)) sg = wg16_min_splits(M, row_bytes);
+    Wg16Group groups[2 * NERF_MAX_LAYERS];
+    const int ng = wg16_groups(P, M, jobs, nj, groups);
+    NERF_CHECK_ARG(ng > 0, "nerf_mlp_weight_grads: layer too wide for the bf16 weight GEMM");
+    for (int gi = 0; gi < ng; ++gi) {
+      const Wg16Group& G = groups[gi];
+      const int n = G.n;
+      const int64_t sg = G.sg;
       Wgrad16Args b;
       b.X = reinterpret_cast<const __bf16*>(stash);
       b.ldx = P.stash_per_sample;
@@ -1014,7 +1053,7 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
       b.njobs = n;
       float* wp = part;
       for (int i = 0; i < n; ++i) {
-        const WgradJob& J = jobs[members[i]];
+        const WgradJob& J = jobs[G.members[i]];
         b.job[i].xoff = J.xoff;
         b.job[i].zoff = J.zoff;
         b.job[i].K = J.K;
@@ -1024,14 +1063,13 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
         b.job[i].bpart = J.bias ? wp : nullptr;
         if (J.bias) wp += sg * J.N;
       }
-      NERF_CHECK_ARG(wp - part <= wgrad_workspace_floats(P, M), "nerf_mlp_weight_grads: workspace layout");
-      launch_wgrad_bf16(b, kb, nb, s);
+      launch_wgrad_bf16(b, G.kb, G.nb, s);
       ReduceBatch rbt;
       rbt.njobs = n;
       rbt.first[0] = 0;
       for (int i = 0; i < n; ++i) {
-        rbt.job[i] = reduce_args(jobs[members[i]], b.job[i].part, b.job[i].bpart, (int)sg);
-        rbt.first[i + 1] = rbt.first[i] + reduce_blocks(jobs[members[i]]);
+        rbt.job[i] = reduce_args(jobs[G.members[i]], b.job[i].part, b.job[i].bpart, (int)sg);
+        rbt.first[i + 1] = rbt.first[i] + reduce_blocks(jobs[G.members[i]]);
       }
       wgrad_reduce_batch_kernel<<<(unsigned)rbt.first[n], 256, 0, s>>>(rbt);
     }
@@ -1058,9 +1096,41 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
       a.bpart = bpart;
       const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * ((splits + 7) / 8) * 8;
       wgrad_gemm_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
-      reduce(J, part, bpart, (int)splits);
+      wgrad_reduce_kernel<<<(unsigned)reduce_blocks(J), 256, 0, s>>>(reduce_args(J, part, bpart, (int)splits));
     }
   }
   NERF_LAUNCH_CHECK("nerf_mlp_weight_grads");
   return NERF_OK;
+}
+
+// M == 0: the gradient of an empty batch is zero (a fill kernel, no memset node
+// in a graph), or left as is when accumulating.
+static int32_t wgrad_empty(const MlpPlan& P, float* grad_flat, int32_t accumulate, hipStream_t s) {
+  if (!accumulate)
+    zero_kernel<<<(unsigned)((P.n_params + 255) / 256 < 4096 ? (P.n_params + 255) / 256 : 4096), 256, 0, s>>>(
+        grad_flat, P.n_params);
+  NERF_LAUNCH_CHECK("nerf_mlp_weight_grads");
+  return NERF_OK;
+}
+
+extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
+                                         const void* stash, const void* dz, float* grad_flat,
+                                         int32_t accumulate, void* workspace,
+                                         int64_t workspace_bytes, void* stream) {
+  MlpPlan P;
+  NERF_CHECK_ARG(nerf_plan_from_desc(desc, precision, &P) == 0,
+                 "nerf_mlp_weight_grads: unsupported MLP description or precision");
+  NERF_CHECK_ARG(M >= 0 && grad_flat, "nerf_mlp_weight_grads: bad arguments");
+  hipStream_t s = as_stream(stream);
+  if (M == 0) return wgrad_empty(P, grad_flat, accumulate, s);
+  NERF_CHECK_ARG(stash && dz && workspace, "nerf_mlp_weight_grads: bad arguments");
+  const int64_t need = wgrad_used_floats(P, precision, M);
+  NERF_CHECK_ARG(need >= 0, "nerf_mlp_weight_grads: layer too wide for the bf16 weight GEMM");
+  NERF_CHECK_ARG(workspace_bytes >= need * 4,
+                 "nerf_mlp_weight_grads: workspace of %lld bytes, this call needs %lld "
+                 "(nerf_mlp_wgrad_workspace_bytes gives %lld for M = %lld)",
+                 (long long)workspace_bytes, (long long)(need * 4),
+                 (long long)(wgrad_workspace_floats(P, M) * 4 + 256), (long long)M);
+  return wgrad_run(P, precision, M, stash, dz, grad_flat, accumulate,
+                   reinterpret_cast<float*>(workspace), s);
 }

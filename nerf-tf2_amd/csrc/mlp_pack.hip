@@ -97,7 +97,7 @@ __global__ void pack_kernel(MlpPlan P, const float* __restrict__ flat, char* __r
     for (int64_t e = tid; e < n; e += nthr)
       o[e] = e < P.fan_out[kl] ? flat[P.b_off[kl] + e] : 0.f;
   } else if (seg == 3 * nm) {
-    const int kl = P.D;  // sigma
+    const int kl = P.kl_sigma;
     float* o = reinterpret_cast<float*>(out + P.p_head_sigma);
     const int KH = P.W / 2;
     for (int64_t e = tid; e < 2 * KH + 4; e += nthr) {
@@ -109,7 +109,7 @@ __global__ void pack_kernel(MlpPlan P, const float* __restrict__ flat, char* __r
       }
     }
   } else if (P.rgb) {
-    const int kl = P.D + 3;  // rgb
+    const int kl = P.kl_rgb;
     float* o = reinterpret_cast<float*>(out + P.p_head_rgb);
     const int KS = 16 * P.NBR;
     for (int64_t e = tid; e < 2 * KS * 4 + 4; e += nthr) {

@@ -52,6 +52,7 @@ struct MlpPlan {
   int KX16, KD16;       // bf16 k-steps for the encodings
   int FX, FD;           // encoding widths (3 + 6L)
   int n_keras;          // Keras layers in trainable-variable order
+  int kl_sigma, kl_bott, kl_rbr, kl_rgb;  // their Keras indices (-1: absent)
   int fan_in[NERF_MAX_LAYERS], fan_out[NERF_MAX_LAYERS];
   int64_t k_off[NERF_MAX_LAYERS], b_off[NERF_MAX_LAYERS];
   int64_t n_params;
@@ -97,7 +98,13 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
   P.KD16 = P.KD / 8;
   P.FX = 3 + 6 * LX;
   P.FD = LD ? 3 + 6 * LD : 0;
-  // Keras layers
+  // Keras layers in trainable-variable order.  A functional model lists its
+  // layers by decreasing depth from the outputs, ties in the order a depth-first
+  // walk from the outputs [rgb, sigma] first meets them (model.py:389-392).  The
+  // last trunk layer feeds the bottleneck (depth 3 below rgb), so it sits at
+  // depth 4; sigma and rgb are both outputs (depth 0) and the walk meets rgb
+  // first: dense_0 .. dense_{D-1}, dense_D, dense_{D+1}, rgb, sigma.  A
+  // density-only net is dense_0 .. dense_{D-1}, sigma.
   int n = 0;
   const int trunk_out_in = W + (skip == D - 1 ? P.FX : 0);
   for (int i = 0; i < D; ++i) {
@@ -105,12 +112,13 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
     P.fan_out[n] = W;
     ++n;
   }
-  P.fan_in[n] = trunk_out_in; P.fan_out[n] = 1; ++n;  // sigma
+  P.kl_bott = P.kl_rbr = P.kl_rgb = -1;
   if (P.rgb) {
-    P.fan_in[n] = trunk_out_in; P.fan_out[n] = W; ++n;        // dense_D (bottleneck)
-    P.fan_in[n] = W + P.FD; P.fan_out[n] = P.WR; ++n;         // dense_{D+1}
-    P.fan_in[n] = P.WR; P.fan_out[n] = 3; ++n;                // rgb
+    P.kl_bott = n; P.fan_in[n] = trunk_out_in; P.fan_out[n] = W; ++n;   // dense_D (bottleneck)
+    P.kl_rbr = n; P.fan_in[n] = W + P.FD; P.fan_out[n] = P.WR; ++n;     // dense_{D+1}
+    P.kl_rgb = n; P.fan_in[n] = P.WR; P.fan_out[n] = 3; ++n;            // rgb
   }
+  P.kl_sigma = n; P.fan_in[n] = trunk_out_in; P.fan_out[n] = 1; ++n;  // sigma
   P.n_keras = n;
   int64_t off = 0;
   for (int i = 0; i < n; ++i) {
@@ -132,14 +140,14 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
     ++m;
   }
   if (P.rgb) {
-    P.keras_of[m] = D + 1;
+    P.keras_of[m] = P.kl_bott;
     P.ks_hid[m] = W / 2;
     P.ks_enc[m] = skip == D - 1 ? P.KX : 0;
     P.enc_L[m] = LX;
     P.nb_out[m] = P.NB;
     P.relu[m] = 0;
     ++m;
-    P.keras_of[m] = D + 2;
+    P.keras_of[m] = P.kl_rbr;
     P.ks_hid[m] = W / 2;
     P.ks_enc[m] = P.KD;
     P.enc_L[m] = LD;

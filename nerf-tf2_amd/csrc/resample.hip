@@ -11,6 +11,13 @@
 // searchsorted(side='right') is an upper_bound binary search; the final
 // tf.sort(concat(t_coarse, t_fine)) is a rank merge (coarse values first on
 // ties), so equal values land in equal slots and the output is the sorted set.
+//
+// Compensated mode (NERF_CDF_COMPENSATED, the BF16 and learned forms): the
+// products w' * width are split exactly into (hi, lo) pairs (fma residual),
+// prefix-summed as compensated pairs (TwoSum), denom is the rounded total and
+// cdf_k = S_k / denom with the division's remainder carried (fma), so every
+// cdf value is within about one ulp of the exact prefix ratio instead of
+// accumulating one rounding per bin.  pdf = w' / denom as in the reference.
 #include "common.h"
 #include <type_traits>
 
@@ -26,12 +33,31 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Compensated inclusive prefix of (ph, pl) pairs over lanes 0..63 (lanes past
+// the bins hold zeros): Hillis-Steele steps with compensated pair adds.
+__device__ __forceinline__ void comp_prefix(float& sh, float& sl, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float oh = shfl_up(sh, d), ol = shfl_up(sl, d);
+    if (lane >= d) dd_add(oh, ol, sh, sl, sh, sl);
+  }
+}
+
+// (sh + sl) / denom rounded once up to the division's own error: the exact
+// remainder of the first quotient is carried into a correction term.
+__device__ __forceinline__ float comp_ratio(float sh, float sl, float denom) {
+  const float q = fdiv(sh, denom);
+  const float r = __builtin_fmaf(-q, denom, sh);
+  return fadd(q, fdiv(fadd(r, sl), denom));
+}
+
 __global__ __launch_bounds__(256) void fine_kernel(
     int64_t n_rays, int nc, int nf, const float* __restrict__ ro, const float* __restrict__ rd,
     const float* __restrict__ bin_weights, const float* __restrict__ bin_edges,
     const float* __restrict__ bin_widths, const float* __restrict__ t_c,
     const float* __restrict__ u_vals, float* __restrict__ t_out, int32_t* __restrict__ idx_out,
-    float* __restrict__ tf_out, float* __restrict__ xyz, float* __restrict__ dirs) {
+    float* __restrict__ tf_out, float* __restrict__ xyz, float* __restrict__ dirs, int comp,
+    float* __restrict__ cdf_out) {
   __shared__ float s_agg[kWaves][kMaxNc + 1];
   __shared__ float s_pdf[kWaves][kMaxNc];
   __shared__ float s_left[kWaves][kMaxNc];
@@ -50,22 +76,35 @@ __global__ __launch_bounds__(256) void fine_kernel(
   const float left = on ? bin_edges[ray * (nc + 1) + lane] : 0.f;
   const float wp = fadd(wv, 1e-5f);      // bin_weights + 1e-5
   const float prod = fmul(wp, width);    // bin_weights * bin_widths
-  const int nv = (nc / 8) * 8;
-  float p = 0.f;
-  for (int i = (lane & 7); i < nv; i += 8) p = fadd(p, shfl(prod, i));
-  // every lane needs the same shuffles: loop bound is lane-independent
-  float tail = 0.f;
-  for (int i = nv; i < nc; ++i) tail = fadd(tail, shfl(prod, i));
-  const float P0 = shfl(p, 0), P1 = shfl(p, 1), P2 = shfl(p, 2), P3 = shfl(p, 3);
-  const float P4 = shfl(p, 4), P5 = shfl(p, 5), P6 = shfl(p, 6), P7 = shfl(p, 7);
-  const float denom =
-      fadd(tail, fadd(fadd(fadd(P0, P4), fadd(P2, P6)), fadd(fadd(P1, P5), fadd(P3, P7))));
-  const float pdf = fdiv(wp, denom);     // ray_utils.py:339
-  const float cw = fmul(pdf, width);     // pdf * bin_widths
-  float agg = 0.f;
-  for (int k = 0; k < nc; ++k) {         // sequential inclusive cumsum (ray_utils.py:342)
-    const float v = shfl(cw, k);
-    if (k <= lane) agg = fadd(agg, v);
+  float pdf, agg;
+  if (comp) {
+    float sh = on ? prod : 0.f, sl = on ? __builtin_fmaf(wp, width, -prod) : 0.f;
+    comp_prefix(sh, sl, lane);
+    const float denom = shfl(sh, nc - 1);
+    pdf = fdiv(wp, denom);
+    agg = comp_ratio(sh, sl, denom);
+  } else {
+    const int nv = (nc / 8) * 8;
+    float p = 0.f;
+    for (int i = (lane & 7); i < nv; i += 8) p = fadd(p, shfl(prod, i));
+    // every lane needs the same shuffles: loop bound is lane-independent
+    float tail = 0.f;
+    for (int i = nv; i < nc; ++i) tail = fadd(tail, shfl(prod, i));
+    const float P0 = shfl(p, 0), P1 = shfl(p, 1), P2 = shfl(p, 2), P3 = shfl(p, 3);
+    const float P4 = shfl(p, 4), P5 = shfl(p, 5), P6 = shfl(p, 6), P7 = shfl(p, 7);
+    const float denom =
+        fadd(tail, fadd(fadd(fadd(P0, P4), fadd(P2, P6)), fadd(fadd(P1, P5), fadd(P3, P7))));
+    pdf = fdiv(wp, denom);               // ray_utils.py:339
+    const float cw = fmul(pdf, width);   // pdf * bin_widths
+    agg = 0.f;
+    for (int k = 0; k < nc; ++k) {       // sequential inclusive cumsum (ray_utils.py:342)
+      const float v = shfl(cw, k);
+      if (k <= lane) agg = fadd(agg, v);
+    }
+  }
+  if (cdf_out) {
+    if (lane == 0) cdf_out[ray * (nc + 1)] = 0.f;
+    if (on) cdf_out[ray * (nc + 1) + 1 + lane] = agg;
   }
   if (lane == 0) s_agg[w][0] = 0.f;      // concat 0 (ray_utils.py:345)
   if (on) {
@@ -191,13 +230,14 @@ __device__ __forceinline__ int count_below(const float* a, float x) {
 // number of fine values < it — coarse first on ties, as the rank merge above), and
 // writes the row from LDS with coalesced stores.  Equal floats are
 // indistinguishable, so the output equals the sorted set bit for bit.
-template <int NC>
+template <int NC, bool COMP>
 __global__ __launch_bounds__(256) void fine128_kernel(
     int64_t n_rays, const float* __restrict__ ro, const float* __restrict__ rd,
     const float* __restrict__ bin_weights, const float* __restrict__ bin_edges,
     const float* __restrict__ bin_widths, const float* __restrict__ t_c,
     const float* __restrict__ u_vals, float* __restrict__ t_out, int32_t* __restrict__ idx_out,
-    float* __restrict__ tf_out, float* __restrict__ xyz, float* __restrict__ dirs) {
+    float* __restrict__ tf_out, float* __restrict__ xyz, float* __restrict__ dirs,
+    float* __restrict__ cdf_out) {
   constexpr int NF = 128, S = NC + NF;
   static_assert(NC >= 2 && NC <= 64, "one coarse bin per lane");
   __shared__ float s_agg[kWaves][NC + 1];
@@ -219,27 +259,40 @@ __global__ __launch_bounds__(256) void fine128_kernel(
   const float tc = on ? t_c[ray * NC + lane] : 0.f;
   const float wp = fadd(wv, 1e-5f);
   const float prod = fmul(wp, width);
-  constexpr int NV = (NC / 8) * 8;
-  // lane j < 8: packet sum P_j = prod_j + prod_{j+8} + ... in that order
-  float pj = prod;
+  float pdf, agg;
+  if constexpr (COMP) {
+    float sh = on ? prod : 0.f, sl = on ? __builtin_fmaf(wp, width, -prod) : 0.f;
+    comp_prefix(sh, sl, lane);
+    const float denom = lane_value(sh, NC - 1);
+    pdf = fdiv(wp, denom);
+    agg = comp_ratio(sh, sl, denom);
+  } else {
+    constexpr int NV = (NC / 8) * 8;
+    // lane j < 8: packet sum P_j = prod_j + prod_{j+8} + ... in that order
+    float pj = prod;
 #pragma unroll
-  for (int i = 1; 8 * i < NV; ++i) pj = fadd(pj, shfl_down(prod, 8 * i));
-  float P[8];
+    for (int i = 1; 8 * i < NV; ++i) pj = fadd(pj, shfl_down(prod, 8 * i));
+    float P[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) P[j] = lane_value(pj, j);
-  float tail = 0.f;
+    for (int j = 0; j < 8; ++j) P[j] = lane_value(pj, j);
+    float tail = 0.f;
 #pragma unroll
-  for (int i = NV; i < NC; ++i) tail = fadd(tail, lane_value(prod, i));
-  const float denom =
-      fadd(tail, fadd(fadd(fadd(P[0], P[4]), fadd(P[2], P[6])), fadd(fadd(P[1], P[5]), fadd(P[3], P[7]))));
-  const float pdf = fdiv(wp, denom);
-  const float cw = fmul(pdf, width);
-  // sequential inclusive prefix: after step s lane i holds cw_{i-s} + ... + cw_i added
-  // left to right (lane -1 reads 0), so after NC - 1 steps every lane < NC holds
-  // ((cw_0 + cw_1) + ...) + cw_i exactly
-  float agg = cw;
+    for (int i = NV; i < NC; ++i) tail = fadd(tail, lane_value(prod, i));
+    const float denom =
+        fadd(tail, fadd(fadd(fadd(P[0], P[4]), fadd(P[2], P[6])), fadd(fadd(P[1], P[5]), fadd(P[3], P[7]))));
+    pdf = fdiv(wp, denom);
+    const float cw = fmul(pdf, width);
+    // sequential inclusive prefix: after step s lane i holds cw_{i-s} + ... + cw_i added
+    // left to right (lane -1 reads 0), so after NC - 1 steps every lane < NC holds
+    // ((cw_0 + cw_1) + ...) + cw_i exactly
+    agg = cw;
 #pragma unroll
-  for (int step = 1; step < NC; ++step) agg = fadd(lane_shift_up(agg), cw);
+    for (int step = 1; step < NC; ++step) agg = fadd(lane_shift_up(agg), cw);
+  }
+  if (cdf_out) {
+    if (lane == 0) cdf_out[ray * (NC + 1)] = 0.f;
+    if (on) cdf_out[ray * (NC + 1) + 1 + lane] = agg;
+  }
   if (lane == 0) s_agg[w][0] = 0.f;
   if (on) {
     s_agg[w][lane + 1] = agg;
@@ -345,11 +398,14 @@ extern "C" int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse
                                                 const float* t_vals_coarse, const float* u_vals,
                                                 float* t_vals, int32_t* piece_idxs,
                                                 float* t_vals_fine, float* xyz_inputs,
-                                                float* dir_inputs, void* stream) {
+                                                float* dir_inputs, int32_t cdf_mode, float* cdf,
+                                                void* stream) {
   NERF_CHECK_ARG(n_rays >= 0 && n_coarse >= 2 && n_coarse <= kMaxNc && n_fine >= 1 &&
                      n_fine <= kMaxNf,
                  "nerf_create_input_batch_fine: n_coarse=%d (2..%d) n_fine=%d (1..%d)", n_coarse,
                  kMaxNc, n_fine, kMaxNf);
+  NERF_CHECK_ARG(cdf_mode == NERF_CDF_TF_ORDER || cdf_mode == NERF_CDF_COMPENSATED,
+                 "nerf_create_input_batch_fine: cdf_mode=%d", cdf_mode);
   if (n_rays == 0) return NERF_OK;  // empty batch
   NERF_CHECK_ARG(bin_weights && bin_edges && bin_widths && t_vals_coarse && u_vals && t_vals,
                  "nerf_create_input_batch_fine: null pointer");
@@ -357,20 +413,24 @@ extern "C" int32_t nerf_create_input_batch_fine(int64_t n_rays, int32_t n_coarse
                  "nerf_create_input_batch_fine: xyz_inputs and dir_inputs go together");
   NERF_CHECK_ARG(!xyz_inputs || (rays_o && rays_d),
                  "nerf_create_input_batch_fine: xyz_inputs needs rays");
-  if (n_rays == 0) return NERF_OK;
   const unsigned grid = (unsigned)((n_rays + kWaves - 1) / kWaves);
-  if (n_fine == 128 && n_coarse == 64)
-    fine128_kernel<64><<<grid, 64 * kWaves, 0, as_stream(stream)>>>(
-        n_rays, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse, u_vals, t_vals,
-        piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
-  else if (n_fine == 128 && n_coarse == 48)
-    fine128_kernel<48><<<grid, 64 * kWaves, 0, as_stream(stream)>>>(
-        n_rays, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse, u_vals, t_vals,
-        piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
-  else
-    fine_kernel<<<grid, 64 * kWaves, 0, as_stream(stream)>>>(
+  hipStream_t s = as_stream(stream);
+  const bool comp = cdf_mode == NERF_CDF_COMPENSATED;
+#define NERF_FINE128(NC_, C_)                                                                    \
+  fine128_kernel<NC_, C_><<<grid, 64 * kWaves, 0, s>>>(n_rays, rays_o, rays_d, bin_weights,      \
+                                                      bin_edges, bin_widths, t_vals_coarse,      \
+                                                      u_vals, t_vals, piece_idxs, t_vals_fine,   \
+                                                      xyz_inputs, dir_inputs, cdf)
+  if (n_fine == 128 && n_coarse == 64) {
+    if (comp) NERF_FINE128(64, true); else NERF_FINE128(64, false);
+  } else if (n_fine == 128 && n_coarse == 48) {
+    if (comp) NERF_FINE128(48, true); else NERF_FINE128(48, false);
+  } else {
+    fine_kernel<<<grid, 64 * kWaves, 0, s>>>(
         n_rays, n_coarse, n_fine, rays_o, rays_d, bin_weights, bin_edges, bin_widths, t_vals_coarse,
-        u_vals, t_vals, piece_idxs, t_vals_fine, xyz_inputs, dir_inputs);
+        u_vals, t_vals, piece_idxs, t_vals_fine, xyz_inputs, dir_inputs, comp ? 1 : 0, cdf);
+  }
+#undef NERF_FINE128
   NERF_LAUNCH_CHECK("nerf_create_input_batch_fine");
   return NERF_OK;
 }

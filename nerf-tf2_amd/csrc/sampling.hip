@@ -19,7 +19,7 @@ void nerf_set_error(const char* fmt, ...) {
 }
 
 extern "C" const char* nerf_last_error(void) { return g_err; }
-extern "C" int32_t nerf_abi_version(void) { return 1; }
+extern "C" int32_t nerf_abi_version(void) { return 2; }
 
 // ---------------------------------------------------------------------------
 // get_rays: float64 pinhole rays, cast to float32.

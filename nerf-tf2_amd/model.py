@@ -41,22 +41,74 @@ class MLPConfig:
         return _lib.desc(self.depth, self.width, self.skip, self.l_xyz, self.l_dir)
 
     def layer_shapes(self) -> List[Tuple[str, int, int]]:
-        """(name, fan_in, fan_out) in Keras trainable-variable order."""
+        """(name, fan_in, fan_out) in Keras trainable-variable order.
+
+        A Keras functional model (model.py:389-392) lists its layers by
+        decreasing depth from the outputs, ties in the order a depth-first walk
+        from the outputs ``[rgb, sigma]`` first meets them.  The last trunk layer
+        feeds the bottleneck, so it is deeper than the sigma head, and sigma and
+        rgb are both outputs (depth 0) with rgb met first: dense_0 .. dense_7,
+        dense_8, dense_9, rgb, sigma.  A density-only net ends with sigma."""
         fx = 3 + 6 * self.l_xyz
         fd = 3 + 6 * self.l_dir
         out, fan = [], fx
         for i in range(self.depth):
             out.append((f"dense_{i}", fan, self.width))
             fan = self.width + (fx if i == self.skip else 0)
-        out.append(("sigma", fan, 1))
         if self.l_dir:
             out.append((f"dense_{self.depth}", fan, self.width))
             out.append((f"dense_{self.depth + 1}", self.width + fd, self.width // 2))
             out.append(("rgb", self.width // 2, 3))
+        out.append(("sigma", fan, 1))
         return out
 
     def macs_per_sample(self) -> int:
         return sum(i * o for _, i, o in self.layer_shapes())
+
+
+def keras_variables(config: MLPConfig, model_name: str) -> List[Tuple[str, Tuple[int, ...]]]:
+    """[(name, shape)] of one model's Keras variables in trainable-variable order:
+    `coarse/dense_0/kernel:0` (63, 256), `coarse/dense_0/bias:0` (256,), ..."""
+    out = []
+    for name, fi, fo in config.layer_shapes():
+        out.append((f"{model_name}/{name}/kernel:0", (fi, fo)))
+        out.append((f"{model_name}/{name}/bias:0", (fo,)))
+    return out
+
+
+def keras_flat_weights(config: MLPConfig, model_name: str, weights) -> np.ndarray:
+    """The flat float32 parameter vector of `weights`, which is either the list
+    Keras' `model.get_weights()` returns (trainable-variable order: dense_0 ..
+    dense_9, rgb, sigma; kernel then bias) or a mapping read by variable name:
+    the full name (`coarse/dense_0/kernel:0`) or the same without the model
+    prefix and/or the `:0` (`dense_0/kernel`), so a file written in any order
+    loads correctly.  Every array is checked against its shape; ValueError on a
+    wrong count, a missing variable or a wrong shape."""
+    variables = keras_variables(config, model_name)
+    order = ", ".join(n for n, _, _ in config.layer_shapes())
+    if hasattr(weights, "keys"):
+        arrays = [_lookup_variable(weights, n) for n, _ in variables]
+    else:
+        arrays = list(weights)
+        if len(arrays) != len(variables):
+            raise ValueError(f"expected {len(variables)} arrays (Keras order {order}), got {len(arrays)}")
+    flat = []
+    for (n, shape), a in zip(variables, arrays):
+        a = np.asarray(a, dtype=np.float32)
+        if tuple(a.shape) != shape:
+            raise ValueError(f"{n}: shape {tuple(a.shape)}, expected {shape} (Keras trainable-variable order: "
+                             f"{order})")
+        flat.append(a.reshape(-1))
+    return np.concatenate(flat)
+
+
+def _lookup_variable(mapping, full: str):
+    bare = full[:-2] if full.endswith(":0") else full
+    local = bare.split("/", 1)[1]
+    for key in (full, bare, local + ":0", local):
+        if key in mapping:
+            return mapping[key]
+    raise ValueError(f"no variable {full!r} (or {local!r}) among the given weights")
 
 
 PRECISIONS = {"fp32": _lib.PREC_F32, "bf16": _lib.PREC_BF16}
@@ -163,17 +215,21 @@ class NerfMLP:
             flat.append(np.zeros((fo,), dtype=np.float32))
         self.params.copy_(torch.from_numpy(np.concatenate(flat)))
 
+    def variable_names(self) -> List[str]:
+        """Keras variable names in trainable-variable order: `coarse/dense_0/kernel:0`, ..."""
+        return [n for n, _ in keras_variables(self.config, self.name)]
+
     def get_weights(self) -> List[np.ndarray]:
-        """Keras-style list [kernel_0, bias_0, ...] as numpy arrays."""
+        """Keras `model.get_weights()`: [kernel, bias] per layer in trainable-variable order."""
         out = []
         for _, k, b in self.layer_slices():
             out += [k.detach().cpu().numpy(), b.detach().cpu().numpy()]
         return out
 
-    def set_weights(self, weights: Sequence) -> None:
-        flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in weights])
-        if flat.size != self.n_params:
-            raise ValueError(f"expected {self.n_params} values, got {flat.size}")
+    def set_weights(self, weights) -> None:
+        """Keras `model.set_weights(list)`, or a name -> array mapping; see
+        keras_flat_weights for what is accepted and checked."""
+        flat = keras_flat_weights(self.config, self.name, weights)
         self.params.copy_(torch.from_numpy(flat))
         self.repack()
 
@@ -222,7 +278,7 @@ class NerfMLP:
             ws = int(_lib.load().nerf_mlp_wgrad_workspace_bytes(ctypes_ref(self._desc), M))
             work = torch.empty(max(ws, 1), dtype=torch.uint8, device=self.device)
         call("nerf_mlp_weight_grads", ctypes_ref(self._desc), self.prec, M, ptr(stash), ptr(dz), ptr(grad),
-             int(accumulate), ptr(work), stream(self.device))
+             int(accumulate), ptr(work), work.numel(), stream(self.device))
 
     def backward_to_params(self, M, g_rgb, g_sigma, rgb, sigma, stash) -> Tensor:
         grad = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
@@ -372,11 +428,122 @@ class DynamicLossScaler:
 
 
 # ---------------------------------------------------------------------------
+# The Keras step contract (model.py:36-55, 127-237) and its metric (ops.py:197-247)
+# ---------------------------------------------------------------------------
+class PSNRMetric:
+    """ops.PSNRMetric (nerf/core/ops.py:197-247), the metric NeRF is compiled
+    with (model.py:408-424): running sums of the squared error and of the count,
+    psnr = -10 log10(sq_error / count).  As in the reference, `count` adds
+    y_true's number of ROWS (rays, ops.py:217), not its elements, so the value is
+    10 log10(3) = 4.77 dB below the per-element PSNR of an rgb batch.  The sums
+    stay on the device; result() returns a 0-d device tensor (no host sync)."""
+
+    name = "psnr_metric"
+
+    def __init__(self, device):
+        self.sq_error = torch.zeros((), dtype=torch.float32, device=device)
+        self.count = torch.zeros((), dtype=torch.float32, device=device)
+
+    def update_state(self, y_true: Tensor, y_pred: Tensor) -> None:
+        self.sq_error += torch.sum(torch.square(y_true - y_pred))
+        self.count += float(y_true.shape[0])
+
+    def result(self) -> Tensor:
+        mse = self.sq_error / self.count
+        return -10.0 * (torch.log(mse) / math.log(10.0))
+
+    def reset_states(self) -> None:
+        self.sq_error.zero_()
+        self.count.zero_()
+
+
+def unpack_data(data):
+    """The reference's dataset element ((rays_o, rays_d, near, far), (rgb,))
+    (model.py:145, 198): returns the 4 ray tensors and rgb (None if absent)."""
+    if not isinstance(data, (tuple, list)) or len(data) not in (1, 2):
+        raise ValueError("data must be ((rays_o, rays_d, near, far), (rgb,))")
+    rays = tuple(data[0])
+    if len(rays) != 4:
+        raise ValueError(f"data[0] must hold rays_o, rays_d, near, far; got {len(rays)} tensors")
+    rgb = None
+    if len(data) == 2 and data[1] is not None:
+        targets = tuple(data[1]) if isinstance(data[1], (tuple, list)) else (data[1],)
+        if len(targets) != 1:
+            raise ValueError("data[1] must be (rgb,)")
+        rgb = targets[0]
+    return rays, rgb
+
+
+class KerasSteps:
+    """`call` / `train_step` / `test_step` / `predict_step` with the reference's
+    signatures (model.py:36-55, 127-237) on top of a class that provides
+    forward(rays_o, rays_d, near, far) and a flat train step.
+
+    * call(inputs) / net(inputs): inputs[0] = (rays_o, rays_d, near, far) ->
+      (post-processed coarse output, post-processed fine output).
+    * train_step(data): data = ((rays_o, rays_d, near, far), (rgb,)); one step,
+      then the compiled metric on the fine prediction; returns
+      {"psnr_metric": value} (0-d device tensor).  The metric accumulates until
+      reset_metrics(), as Keras does over an epoch.
+    * test_step(data): forward without gradients, metric update, same dict.
+    * predict_step(data): call(data) without gradients.
+    train_step with the tensors as separate arguments is the flat form
+    (returns the losses, no metric)."""
+
+    def _psnr(self) -> PSNRMetric:
+        m = getattr(self, "_psnr_metric", None)
+        if m is None:
+            m = self._psnr_metric = PSNRMetric(self.device)
+        return m
+
+    @property
+    def metrics(self):
+        return [self._psnr()]
+
+    def reset_metrics(self) -> None:
+        self._psnr().reset_states()
+
+    def _metrics_dict(self) -> Dict[str, Tensor]:
+        return {m.name: m.result() for m in self.metrics}
+
+    def call(self, inputs):
+        out = self.forward(*unpack_data(inputs)[0])
+        return out[0], out[1]
+
+    def __call__(self, inputs):
+        return self.call(inputs)
+
+    def train_step(self, *args, **kwargs) -> Dict[str, Tensor]:
+        if len(args) == 1 and not kwargs:
+            (rays_o, rays_d, near, far), rgb = unpack_data(args[0])
+            if rgb is None:
+                raise ValueError("train_step(data) needs data[1] = (rgb,)")
+            out = self._train_step_flat(rays_o, rays_d, near, far, rgb)
+            self._psnr().update_state(rgb, out["pred_rgb"])
+            return self._metrics_dict()
+        return self._train_step_flat(*args, **kwargs)
+
+    @torch.no_grad()
+    def test_step(self, data) -> Dict[str, Tensor]:
+        rays, rgb = unpack_data(data)
+        if rgb is None:
+            raise ValueError("test_step(data) needs data[1] = (rgb,)")
+        _, pp_f = self.call((rays,))
+        self._psnr().update_state(rgb, pp_f["pred_rgb"])
+        return self._metrics_dict()
+
+    @torch.no_grad()
+    def predict_step(self, data):
+        return self.call(data)
+
+
+# ---------------------------------------------------------------------------
 # NeRF (model.py:18-237)
 # ---------------------------------------------------------------------------
 def coarse_fine_forward(params, coarse: NerfMLP, fine: NerfMLP, rays_o, rays_d, near, far, white_bg: bool,
-                        u_coarse=None, u_fine=None, return_indices=False):
-    """NeRF.forward (model.py:57-125) on the fused kernels."""
+                        u_coarse=None, u_fine=None, return_indices=False, compensated=False):
+    """NeRF.forward (model.py:57-125) on the fused kernels.  compensated: the
+    resampler's cdf with compensated summation (BF16 and learned forms)."""
     data_CM = ray_utils.create_input_batch_coarse_model(params, rays_o, rays_d, near, far,
                                                         u_vals=u_coarse, materialize=False)
     rgb_c, sig_c = coarse.forward_rays(rays_o, rays_d, data_CM["t_vals"])
@@ -384,7 +551,7 @@ def coarse_fine_forward(params, coarse: NerfMLP, fine: NerfMLP, rays_o, rays_d, 
     data_FM = ray_utils.create_input_batch_fine_model(params, rays_o, rays_d, pp_c["weights"],
                                                       data_CM["bin_data"], data_CM["t_vals"],
                                                       u_vals=u_fine, materialize=False,
-                                                      return_indices=return_indices)
+                                                      return_indices=return_indices, compensated=compensated)
     rgb_f, sig_f = fine.forward_rays(rays_o, rays_d, data_FM["t_vals"])
     pp_f = ray_utils.post_process_model_output(rgb_f, sig_f, data_FM["t_vals"], white_bg)
     if return_indices:
@@ -394,7 +561,7 @@ def coarse_fine_forward(params, coarse: NerfMLP, fine: NerfMLP, rays_o, rays_d, 
     return pp_c, pp_f
 
 
-class NeRF:
+class NeRF(KerasSteps):
     """Plain coarse + fine NeRF (model.py:18-237).  precision "fp32" is the
     reference form; "bf16" runs the MLP matmuls in bf16 with fp32 master
     weights and dynamic loss scaling."""
@@ -409,6 +576,9 @@ class NeRF:
         self.fine_model = NerfMLP("fine", mlp_config, precision, seed=seed + 1, device=device)
         self.optimizer = AdamTF([self.coarse_model, self.fine_model])
         self.scaler = DynamicLossScaler(self.optimizer) if precision == "bf16" else None
+        # the plain FP32 form keeps the reference's cdf order (bit-exact bins);
+        # the precision form sums the cdf with compensation
+        self.compensated_cdf = precision != "fp32"
 
     @property
     def models(self):
@@ -417,7 +587,8 @@ class NeRF:
     def forward(self, rays_o, rays_d, near, far, u_coarse=None, u_fine=None, return_indices=False):
         """model.py:57-125 -> (post_proc_CM, post_proc_FM)."""
         return coarse_fine_forward(self.params, self.coarse_model, self.fine_model, rays_o, rays_d, near, far,
-                                   self.white_bg, u_coarse, u_fine, return_indices)
+                                   self.white_bg, u_coarse, u_fine, return_indices,
+                                   compensated=self.compensated_cdf)
 
     @torch.no_grad()
     def predict(self, rays_o, rays_d, near, far, chunk: int = 32768, keep_weights: bool = False):
@@ -449,7 +620,7 @@ class NeRF:
                 mdl.params.requires_grad_(False)
             self.optimizer.apply_gradients(grads, self.scaler)
 
-    def train_step(self, rays_o, rays_d, near, far, rgb, u_coarse=None, u_fine=None) -> Dict[str, Tensor]:
+    def _train_step_flat(self, rays_o, rays_d, near, far, rgb, u_coarse=None, u_fine=None) -> Dict[str, Tensor]:
         """model.py:127-180: MSE(coarse) + MSE(fine), backward, Adam step."""
         for mdl in self.models:
             mdl.params.requires_grad_(True)

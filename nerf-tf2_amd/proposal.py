@@ -28,7 +28,7 @@ from typing import Dict, Optional
 import torch
 
 from . import ray_utils
-from .model import AdamTF, DynamicLossScaler, MLPConfig, NerfMLP, coarse_fine_forward
+from .model import AdamTF, DynamicLossScaler, KerasSteps, MLPConfig, NerfMLP, coarse_fine_forward
 
 Tensor = torch.Tensor
 
@@ -48,7 +48,7 @@ class _P:
         self.system.white_bg = white_bg
 
 
-class ProposalNeRF:
+class ProposalNeRF(KerasSteps):
     def __init__(self, params, precision: str = "bf16", seed: int = 0, device="cuda", n_prop: int = 48,
                  fallback_threshold: float = 1e-3, coarse_stride: int = 8, interlevel_weight: float = 10.0,
                  dilate: int = 2, proposal_config: MLPConfig = PROPOSAL_CONFIG, sample_floor: float = 0.02):
@@ -89,7 +89,8 @@ class ProposalNeRF:
         if self.sample_floor > 0.0:
             w_s = w_s + (self.sample_floor / self.n_prop) * w_s.sum(dim=1, keepdim=True)
         df = ray_utils.create_input_batch_fine_model(self.p_prop, rays_o, rays_d, w_s, dp["bin_data"],
-                                                     dp["t_vals"], u_vals=u_fine, materialize=False)
+                                                     dp["t_vals"], u_vals=u_fine, materialize=False,
+                                                     compensated=True)
         rgb_f, sig_f = self.fine_model.forward_rays(rays_o, rays_d, df["t_vals"])
         pp_f = ray_utils.post_process_model_output(rgb_f, sig_f, df["t_vals"], self.white_bg)
         return dp, pp_p, df, pp_f
@@ -109,7 +110,7 @@ class ProposalNeRF:
                 o, d = rays_o.index_select(0, idx), rays_d.index_select(0, idx)
                 nr, fr = near.index_select(0, idx), far.index_select(0, idx)
                 _, pf2 = coarse_fine_forward(self.params, self.coarse_model, self.fine_model, o, d, nr, fr,
-                                             self.white_bg)
+                                             self.white_bg, compensated=True)
                 for k in ("pred_rgb", "pred_depth", "acc_map"):
                     pp_f[k] = pp_f[k].index_copy(0, idx, pf2[k])
         return pp_p, pp_f, missed
@@ -128,7 +129,8 @@ class ProposalNeRF:
         return {k: torch.cat(v) for k, v in outs.items()}
 
     # ------------------------------------------------------------------
-    def train_step(self, rays_o, rays_d, near, far, rgb, u_prop=None, u_fine=None, u_coarse=None) -> Dict[str, Tensor]:
+    def _train_step_flat(self, rays_o, rays_d, near, far, rgb, u_prop=None, u_fine=None,
+                         u_coarse=None) -> Dict[str, Tensor]:
         """One step; u_prop (N, n_prop), u_fine (N, N_fine) and u_coarse
         (ceil(N / coarse_stride), N_coarse) inject the uniform draws."""
         for mdl in self.models:
@@ -164,7 +166,8 @@ class ProposalNeRF:
             for m in models:
                 m.params.requires_grad_(False)
             self.optimizer.apply_gradients(grads, self.scaler)
-        return {"fine_loss": lf.detach(), "coarse_loss": lc.detach(), "interlevel": il.mean().detach(),
+        return {"fine_loss": lf.detach(), "coarse_loss": lc.detach(), "pred_rgb": pp_f["pred_rgb"].detach(),
+                "interlevel": il.mean().detach(),
                 "missed_mean": missed.mean().detach(),
                 "fallback_fraction": (missed > self.fallback_threshold).float().mean().detach()}
 

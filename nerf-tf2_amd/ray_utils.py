@@ -23,6 +23,8 @@ from ._lib import call, ptr, stream
 
 Tensor = torch.Tensor
 
+CDF_TF_ORDER, CDF_COMPENSATED = 0, 1     # nerf_create_input_batch_fine's cdf_mode
+
 
 def _f32(t: Tensor) -> Tensor:
     if t.dtype != torch.float32:
@@ -92,7 +94,12 @@ def create_input_batch_coarse_model(params, rays_o: Tensor, rays_d: Tensor, near
 # ---------------------------------------------------------------------------
 def create_input_batch_fine_model(params, rays_o: Tensor, rays_d: Tensor, bin_weights: Tensor,
                                   bin_data: Dict, t_vals_coarse: Tensor, u_vals: Optional[Tensor] = None,
-                                  materialize: bool = True, return_indices: bool = False) -> Dict:
+                                  materialize: bool = True, return_indices: bool = False,
+                                  compensated: bool = False, return_cdf: bool = False) -> Dict:
+    """ray_utils.py:276-406.  compensated=False keeps the reference's float32
+    operation order for the denominator and the cdf (bin indices bit-exact with
+    it); True sums them with compensation (the BF16 and learned forms).
+    return_cdf adds "cdf" (N, Nc+1), the cdf the draws are inverted against."""
     Nc = int(t_vals_coarse.shape[1])
     Nf = int(params.sampling.N_fine)
     n = rays_o.shape[0]
@@ -112,14 +119,18 @@ def create_input_batch_fine_model(params, rays_o: Tensor, rays_d: Tensor, bin_we
     if materialize:
         xyz = torch.empty((n * S, 3), dtype=torch.float32, device=dev)
         dirs = torch.empty_like(xyz)
+    cdf = torch.empty((n, Nc + 1), dtype=torch.float32, device=dev) if return_cdf else None
     call("nerf_create_input_batch_fine", n, Nc, Nf, ptr(_f32(rays_o)), ptr(_f32(rays_d)),
          ptr(_f32(bin_weights.detach())), ptr(_f32(bin_data["bin_edges"])),
          ptr(_f32(bin_data["bin_widths"])), ptr(_f32(t_vals_coarse)), ptr(u_vals), ptr(t_vals),
-         ptr(idx), ptr(tf), ptr(xyz), ptr(dirs), stream(dev))
+         ptr(idx), ptr(tf), ptr(xyz), ptr(dirs), CDF_COMPENSATED if compensated else CDF_TF_ORDER,
+         ptr(cdf), stream(dev))
     out = {"xyz_inputs": xyz, "dir_inputs": dirs, "t_vals": t_vals}
     if return_indices:
         out["piece_idxs"] = idx
         out["t_vals_fine"] = tf
+    if return_cdf:
+        out["cdf"] = cdf
     return out
 
 

@@ -228,10 +228,10 @@ class MLPSpec:
         for i in range(self.D):
             out.append((f"dense_{i}", fan, self.W))
             fan = self.W + (self.enc_xyz if i == self.skip else 0)
-        out.append(("sigma", fan, 1))
         out.append((f"dense_{self.D}", fan, self.W))
         out.append((f"dense_{self.D + 1}", self.W + self.enc_dir, self.W_rgb))
         out.append(("rgb", self.W_rgb, 3))
+        out.append(("sigma", fan, 1))
         return out
 
     def n_params(self) -> int:
@@ -263,11 +263,12 @@ def mlp_forward(spec: MLPSpec, weights: Sequence[Tensor], xyz: Tensor, dirs: Ten
         h = relu(h @ weights[p] + weights[p + 1]); p += 2
         if i == spec.skip:
             h = torch.cat([h, enc_x], dim=-1)
-    sigma = relu(h @ weights[p] + weights[p + 1]); p += 2
-    bott = h @ weights[p] + weights[p + 1]; p += 2
+    trunk = h
+    bott = trunk @ weights[p] + weights[p + 1]; p += 2
     h = torch.cat([bott, enc_d], dim=-1)
     h = relu(h @ weights[p] + weights[p + 1]); p += 2
     rgb = torch.sigmoid(h @ weights[p] + weights[p + 1]); p += 2
+    sigma = relu(trunk @ weights[p] + weights[p + 1]); p += 2
     return rgb, sigma
 
 

@@ -50,7 +50,7 @@ def make():
                w_fine=pf["weights"].detach().numpy(), rgb_fine=pf["pred_rgb"].detach().numpy(),
                depth_fine=pf["pred_depth"].detach().numpy(), acc_fine=pf["acc_map"].detach().numpy(),
                loss_coarse=lc.numpy(), loss_fine=lf.numpy(),
-               g_coarse_dense0=gc[0].numpy(), g_fine_rgb=gf[-2].numpy())
+               g_coarse_dense0=gc[0].numpy(), g_fine_rgb=gf[20].numpy())   # rgb kernel (Keras order: ..., rgb, sigma)
     np.savez_compressed(os.path.join(HERE, "config0_small.npz"), **out)
 
 

@@ -34,7 +34,7 @@ def test_library_exports_header_symbols(nerf):
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(nerf._lib._SIGS), "ctypes signatures out of sync with the header"
-    assert lib.nerf_abi_version() == 1
+    assert lib.nerf_abi_version() == 2
 
 
 def test_size_queries(nerf):
@@ -67,7 +67,7 @@ def test_invalid_arguments_rejected_without_launch(nerf):
     rc = lib.nerf_get_rays(0, 10, None, None, None, None, None)
     assert rc == nerf._lib.NERF_ERR_INVALID_ARG
     assert b"H=0" in lib.nerf_last_error()
-    rc = lib.nerf_create_input_batch_fine(10, 65, 128, *([None] * 12), None)
+    rc = lib.nerf_create_input_batch_fine(10, 65, 128, *([None] * 12), 0, None, None)
     assert rc == nerf._lib.NERF_ERR_INVALID_ARG
     assert b"n_coarse=65" in lib.nerf_last_error()
     rc = lib.nerf_composite_forward(10, 1000, 1, *([None] * 7), None)
@@ -81,8 +81,33 @@ def test_checkpoint_variable_layout(nerf):
     assert len(v) == 24
     assert v[0] == ("coarse/dense_0/kernel", (63, 256)) and v[1] == ("coarse/dense_0/bias", (256,))
     assert v[10] == ("coarse/dense_5/kernel", (319, 256))
-    assert [n for n, _ in v[16:]] == ["coarse/sigma/kernel", "coarse/sigma/bias", "coarse/dense_8/kernel",
-                                      "coarse/dense_8/bias", "coarse/dense_9/kernel", "coarse/dense_9/bias",
-                                      "coarse/rgb/kernel", "coarse/rgb/bias"]
-    assert v[20][1] == (283, 128)
+    assert [n for n, _ in v[16:]] == ["coarse/dense_8/kernel", "coarse/dense_8/bias", "coarse/dense_9/kernel",
+                                      "coarse/dense_9/bias", "coarse/rgb/kernel", "coarse/rgb/bias",
+                                      "coarse/sigma/kernel", "coarse/sigma/bias"]
+    assert v[18][1] == (283, 128) and v[22][1] == (256, 1)
     assert sum(int(np.prod(s)) for _, s in v) == 595844
+
+
+def test_undersized_wgrad_workspace_rejected(nerf):
+    """nerf_mlp_weight_grads checks the caller's workspace size against what the
+    call's launches use, before launching anything (the pointers are never
+    touched on a rejected call, so placeholders stand in for device buffers)."""
+    lib = nerf._lib.load()
+    d = nerf.MLPConfig().desc()
+    fake = ctypes.c_void_p(0x1000)
+    for prec in (nerf._lib.PREC_F32, nerf._lib.PREC_BF16):
+        for M in (1, 4099, 786432):
+            full = lib.nerf_mlp_wgrad_workspace_bytes(ctypes.byref(d), M)
+            rc = lib.nerf_mlp_weight_grads(ctypes.byref(d), prec, M, fake, fake, fake, 0, fake, 0, None)
+            assert rc == nerf._lib.NERF_ERR_INVALID_ARG
+            msg = lib.nerf_last_error().decode()
+            assert "workspace of 0 bytes" in msg and "needs" in msg, msg
+            need = int(msg.split("needs ")[1].split()[0])
+            assert (256 * 256 + 256) * 4 <= need <= full
+            for small in (1024, need - 4):
+                rc = lib.nerf_mlp_weight_grads(ctypes.byref(d), prec, M, fake, fake, fake, 0, fake, small, None)
+                assert rc == nerf._lib.NERF_ERR_INVALID_ARG, (prec, M, small)
+                assert f"needs {need}" in lib.nerf_last_error().decode()
+    # a null workspace is an invalid argument too
+    rc = lib.nerf_mlp_weight_grads(ctypes.byref(d), 0, 10, fake, fake, fake, 0, None, 1 << 30, None)
+    assert rc == nerf._lib.NERF_ERR_INVALID_ARG

@@ -66,10 +66,11 @@ def _bf16_emulated_mlp(oracle, W, xyz, dirs):
         h = torch.relu(lin(h, W[2 * i], W[2 * i + 1]))
         if i == 4:
             h = torch.cat([h, ex], -1)
-    sigma = torch.relu(head(h, W[16], W[17]))
-    bott = lin(h, W[18], W[19])
-    h9 = torch.relu(lin(torch.cat([bott, ed], -1), W[20], W[21]))
-    rgb = torch.sigmoid(head(h9, W[22], W[23]))
+    # Keras order after the trunk: dense_8, dense_9, rgb, sigma
+    bott = lin(h, W[16], W[17])
+    h9 = torch.relu(lin(torch.cat([bott, ed], -1), W[18], W[19]))
+    rgb = torch.sigmoid(head(h9, W[20], W[21]))
+    sigma = torch.relu(head(h, W[22], W[23]))
     return rgb, sigma
 
 

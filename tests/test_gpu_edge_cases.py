@@ -43,10 +43,10 @@ def test_empty_batches(nerf):
         L.call("nerf_mlp_backward", d, mlp.prec, L.ptr(mlp.packed), 0, None, None, None, None, None, None,
                L.stream())
         g = torch.full((mlp.n_params,), 7.0, device=DEV)
-        L.call("nerf_mlp_weight_grads", d, mlp.prec, 0, None, None, L.ptr(g), 1, None, L.stream())
+        L.call("nerf_mlp_weight_grads", d, mlp.prec, 0, None, None, L.ptr(g), 1, None, 0, L.stream())
         torch.cuda.synchronize()
         assert bool((g == 7.0).all())
-        L.call("nerf_mlp_weight_grads", d, mlp.prec, 0, None, None, L.ptr(g), 0, None, L.stream())
+        L.call("nerf_mlp_weight_grads", d, mlp.prec, 0, None, None, L.ptr(g), 0, None, 0, L.stream())
         torch.cuda.synchronize()
         assert bool((g == 0.0).all())
 
@@ -136,7 +136,7 @@ def test_weight_grads_additive(nerf, prec, M, M1):
             ws = torch.empty((max(int(L.load().nerf_mlp_wgrad_workspace_bytes(d, b - a)), 1),), dtype=torch.uint8,
                              device=DEV)
             L.call("nerf_mlp_weight_grads", d, mlp.prec, b - a, L.ptr(stash[a:b]), L.ptr(dz[a:b]), L.ptr(out),
-                   int(i > 0), L.ptr(ws), L.stream())
+                   int(i > 0), L.ptr(ws), ws.numel(), L.stream())
         torch.cuda.synchronize()
         return out.double()
 

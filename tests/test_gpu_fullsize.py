@@ -11,7 +11,7 @@ invariants at this size."""
 import pytest
 import torch
 
-from test_gpu_parity import DEV, _close, _oracle_weights, _rays_batch
+from test_gpu_parity import CDF_EDGE_TOL, DEV, _close, _oracle_weights, _rays_batch, explain_bin_mismatches
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +62,13 @@ def test_plain_fp32_chunk_matches_oracle_on_a_subset(nerf, oracle):
                                                 cpu(dc_gpu["t_vals"]), u_vals=cpu(u_f))
     assert torch.equal(pp_f["piece_idxs"][sd].cpu(), ro_f["piece_idxs"])
     assert torch.equal(pp_f["t_vals"][sd].cpu(), ro_f["t_vals"])
+    # end to end: the launch's bins against the oracle's own coarse + fine pass, every
+    # difference explained by a cdf edge moved across the draw
+    rays, samples, worst, shift = explain_bin_mismatches(
+        oracle, pp_f["piece_idxs"][sd], pp_c["weights"][sd], rc["weights"], dC["bin_data"]["left_edges"],
+        dC["bin_data"]["bin_widths"], cpu(u_f))
+    print(f"\nend-to-end fine bins on the subset: {samples} samples differ on {rays} of {SUBSET} rays, each a "
+          f"draw within {worst:.3g} of a cdf edge; max cdf edge shift {shift:.3g} (bound {CDF_EDGE_TOL:.3g})")
 
     # fine stage at full size, the subset's rows on the oracle's fine sample positions
     t_f = pp_f["t_vals"].clone()

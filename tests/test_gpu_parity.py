@@ -25,6 +25,48 @@ def _close(a, b, rtol=1e-4, floor=1e-6, what=""):
                                  f"atol={atol:.3g}; max abs err {float(err.max()):.3g}")
 
 
+# A draw within this distance of a cdf edge may fall in either neighbouring bin
+# when the coarse weights feeding the cdf differ in their last bits: the plain
+# path's coarse weights agree with the oracle's to ~1e-6 relative (bar 1e-4), and
+# the cdf is a sum of normalised weights, so its edges move by a few 1e-6 at most.
+# 2^-17 = 128 ulps of a value in [0.5, 1).
+CDF_EDGE_TOL = 2.0 ** -17
+
+
+def explain_bin_mismatches(oracle, idx_gpu, w_gpu, w_ref, left_edges, bin_widths, u_f, tol=CDF_EDGE_TOL):
+    """Fine bin indices of a full GPU forward against the oracle's end-to-end
+    ones (ray_utils.py:335-363).  Every difference must be explained:
+      1. the oracle resampler fed the GPU's own coarse weights reproduces every GPU
+         bin bit for bit, so a difference can only come from the coarse weights;
+      2. for each differing (ray, sample), every cdf edge e between the two bins
+         has moved across the draw: searchsorted(side='right') put u on opposite
+         sides of agg_gpu[e] and agg_ref[e] (agg_gpu[e] <= u < agg_ref[e] or the
+         reverse);
+      3. those edges moved by at most `tol`, so |u - agg_ref[e]| <= tol.
+    Returns (rays with a difference, differing samples, max |u - agg_ref[e]|,
+    max |agg_gpu - agg_ref| over every edge)."""
+    idx_gpu = idx_gpu.cpu()
+    u_f = u_f.cpu()
+    _, idx_gw, _, agg_gw = oracle.sample_pdf(w_gpu.detach().cpu(), left_edges.cpu(), bin_widths.cpu(), u_f)
+    assert torch.equal(idx_gw, idx_gpu), "the resampler differs from the oracle on identical coarse weights"
+    _, idx_ref, _, agg_ref = oracle.sample_pdf(w_ref.detach().cpu(), left_edges.cpu(), bin_widths.cpu(), u_f)
+    diff = idx_gpu != idx_ref
+    worst = 0.0
+    for r, q in torch.nonzero(diff).tolist():
+        u = float(u_f[r, q])
+        ig, ir = int(idx_gpu[r, q]), int(idx_ref[r, q])
+        for e in range(min(ig, ir) + 1, max(ig, ir) + 1):
+            ag, ar = float(agg_gw[r, e]), float(agg_ref[r, e])
+            if ig > ir:
+                assert ag <= u < ar, (r, q, e, u, ag, ar)
+            else:
+                assert ar <= u < ag, (r, q, e, u, ag, ar)
+            assert abs(ag - ar) <= tol, (r, q, e, abs(ag - ar))
+            worst = max(worst, abs(u - ar))
+    shift = float((agg_gw.double() - agg_ref.double()).abs().max())
+    return int(diff.any(dim=1).sum()), int(diff.sum()), worst, shift
+
+
 def _rays_batch(nerf, n=1024, seed=0, H=100, W=100, focal=138.9):
     sc = nerf.scene
     rng = np.random.default_rng(seed)
@@ -247,8 +289,10 @@ def test_nerf_forward_config0(nerf, oracle):
     (t = (u - cdf)/pdf + left), and the L=10 encoding turns a 1e-7 shift of a
     sample into a ~1e-3 change of the MLP output, so independently computed
     float32 coarse weights cannot give fine outputs within 1e-4; the fine pass is
-    therefore compared on the oracle's own fine t_vals, and the bin indices end to
-    end (they only differ where a draw lies within rounding of a cdf edge)."""
+    therefore compared on the oracle's own fine t_vals.  The bin indices are
+    compared end to end: every difference must come from a cdf edge that the
+    coarse weights' last-bit differences moved across the draw
+    (explain_bin_mismatches)."""
     n = 1024
     ro, rd, near, far = _rays_batch(nerf, n, seed=0)
     P = nerf.Params()
@@ -263,8 +307,11 @@ def test_nerf_forward_config0(nerf, oracle):
                                     ro.cpu(), rd.cpu(), near.cpu(), far.cpu(), u_c.cpu(), u_f.cpu())
     for k in ("weights", "pred_rgb", "pred_depth", "acc_map"):
         _close(pp_c[k], rc[k], rtol=1e-4, floor=1e-6, what="coarse " + k)
-    mism = (pp_f["piece_idxs"].cpu() != dF["piece_idxs"]).any(dim=1)
-    assert int(mism.sum()) <= 8, f"{int(mism.sum())} rays with differing fine bins"
+    rays, samples, worst, shift = explain_bin_mismatches(
+        oracle, pp_f["piece_idxs"], pp_c["weights"], rc["weights"], dC["bin_data"]["left_edges"],
+        dC["bin_data"]["bin_widths"], u_f)
+    print(f"\nend-to-end fine bins: {samples} of {n * 128} samples differ on {rays} of {n} rays, each a draw "
+          f"within {worst:.3g} of a cdf edge; max cdf edge shift {shift:.3g} (bound {CDF_EDGE_TOL:.3g})")
     # fine stage on the oracle's fine sample positions
     t_f = dF["t_vals"].to(DEV)
     with torch.no_grad():

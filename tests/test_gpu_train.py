@@ -56,11 +56,11 @@ def _relu_margin(oracle, conf, W, xyz, dirs):
         h = torch.relu(z)
         if i == conf.skip:
             h = torch.cat([h, enc_x], dim=-1)
-    if conf.l_dir:
-        z = h @ W[p] + W[p + 1]                      # sigma = ReLU(dense)
+    if conf.l_dir:   # Keras order after the trunk: dense_8, dense_9, rgb, sigma
+        bott = h @ W[p] + W[p + 1]
+        z = torch.cat([bott, oracle.positional_encoding(dirs.double(), conf.l_dir)], dim=-1) @ W[p + 2] + W[p + 3]
         margin = torch.minimum(margin, z.abs().min(dim=1).values)
-        bott = h @ W[p + 2] + W[p + 3]
-        z = torch.cat([bott, oracle.positional_encoding(dirs.double(), conf.l_dir)], dim=-1) @ W[p + 4] + W[p + 5]
+        z = h @ W[p + 6] + W[p + 7]                  # sigma = ReLU(dense)
         margin = torch.minimum(margin, z.abs().min(dim=1).values)
     return margin
 

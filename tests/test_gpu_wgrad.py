@@ -41,7 +41,8 @@ def test_bf16_weight_grads_match_fp32_kernels(nerf, net, M):
         out = torch.empty((mlp.n_params,), device=DEV)
         ws = torch.empty((max(int(L.load().nerf_mlp_wgrad_workspace_bytes(d, M)), 1),), dtype=torch.uint8,
                          device=DEV)
-        L.call("nerf_mlp_weight_grads", d, mlp.prec, M, L.ptr(s), L.ptr(z), L.ptr(out), 0, L.ptr(ws), L.stream())
+        L.call("nerf_mlp_weight_grads", d, mlp.prec, M, L.ptr(s), L.ptr(z), L.ptr(out), 0, L.ptr(ws), ws.numel(),
+               L.stream())
         torch.cuda.synchronize()
         return out.double().cpu()
 

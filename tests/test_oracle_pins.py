@@ -108,15 +108,16 @@ def test_mlp_topology(oracle):
     assert spec.n_params() == 595844
     assert spec.macs_per_sample() == 593408
     shapes = spec.layer_shapes()
-    assert [s[0] for s in shapes] == [f"dense_{i}" for i in range(8)] + ["sigma", "dense_8", "dense_9", "rgb"]
-    assert shapes[5][1] == 256 + 63 and shapes[9][1] == 256 and shapes[10][1] == 256 + 27 and shapes[11][1] == 128
+    assert [s[0] for s in shapes] == [f"dense_{i}" for i in range(10)] + ["rgb", "sigma"]
+    assert shapes[5][1] == 256 + 63 and shapes[8][1] == 256 and shapes[9][1] == 256 + 27 and shapes[10][1] == 128
+    assert shapes[11][1:] == (256, 1)
 
 
 def test_mlp_zero_weight_closed_form(oracle):
     spec = oracle.MLPSpec()
     W = [torch.zeros_like(torch.from_numpy(w)) for w in oracle.glorot_weights(spec, 0)]
-    W[17] = torch.tensor([0.7])                   # sigma bias
-    W[23] = torch.tensor([-1.0, 0.0, 2.0])        # rgb bias
+    W[23] = torch.tensor([0.7])                   # sigma bias
+    W[21] = torch.tensor([-1.0, 0.0, 2.0])        # rgb bias
     xyz = torch.randn(5, 3)
     dirs = torch.randn(5, 3)
     rgb, sigma = oracle.mlp_forward(spec, W, xyz, dirs)
@@ -133,7 +134,7 @@ def test_mlp_skip_and_concat_wiring(oracle):
     dirs = torch.nn.functional.normalize(torch.randn(64, 3, dtype=torch.float64), dim=1)
     rgb, sigma = oracle.mlp_forward(spec, W, xyz, dirs)
     g = torch.autograd.grad(rgb.sum() + sigma.sum(), W)
-    k5, k9 = g[10], g[20]
+    k5, k9 = g[10], g[18]
     assert k5.shape == (319, 256) and float(k5[256:].abs().sum()) > 0
     assert k9.shape == (283, 128) and float(k9[256:].abs().sum()) > 0
     # rows 256.. of dense_5 multiply enc_xyz: its gradient is enc_xyz^T dZ5
@@ -171,9 +172,9 @@ def test_mlp_float64_vs_torch_linear(oracle):
         h = torch.relu(lin[i](h))
         if i == 4:
             h = torch.cat([h, ex], -1)
-    sig = torch.relu(lin[8](h))
-    h = torch.relu(lin[10](torch.cat([lin[9](h), ed], -1)))
-    rgb = torch.sigmoid(lin[11](h))
+    sig = torch.relu(lin[11](h))
+    h = torch.relu(lin[9](torch.cat([lin[8](h), ed], -1)))
+    rgb = torch.sigmoid(lin[10](h))
     rgb_o, sig_o = oracle.mlp_forward(spec, W, xyz, dirs)
     assert torch.allclose(rgb, rgb_o, atol=1e-12) and torch.allclose(sig, sig_o, atol=1e-12)
 
@@ -381,7 +382,9 @@ def test_train_gradients_finite_differences(oracle):
     rng = np.random.default_rng(0)
     checked = 0
     for which, W, G in (("c", wc, gc), ("f", wf, gf)):
-        for li in (0, 2, 8, 9, 11):   # dense_0 kernel, dense_1 kernel, sigma kernel, sigma bias, dense_5? (D=4)
+        # D=4 net in Keras order: dense_0..3 (0-7), dense_4 = bottleneck (8, 9), dense_5 (10, 11),
+        # rgb (12, 13), sigma (14, 15)
+        for li in (0, 2, 8, 11, 14, 15):
             if li >= len(W):
                 continue
             flat = W[li].reshape(-1)
