@@ -10,6 +10,14 @@ Prints ONE JSON line.  `value` is render rays/s of the whole job; train rays/s,
 the roofline of the dominant kernel (the fused fine-model MLP forward, timed
 with HIP events on its stream) and the CPU baseline (the oracle restatement on
 the host cores, bounded sample) ride along.
+
+Quality half of the metric (`learned_vs_plain`): the plain FP32 form and the
+learned BF16 + proposal form are trained for --quality-steps steps (2000) on the
+same 4096-ray batches (distinct batches, float64 ground truth) from the same
+initial weights, then render --quality-views held-out 800x800 views; PSNR and
+max |rgb error| against the float64 ground truth are reported for the final
+weights and for the weights averaged over the last --quality-average-last steps.
+Their train rays/s are timed over those steps.
 """
 from __future__ import annotations
 
@@ -46,7 +54,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--train-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="bound on the CPU baseline's render sample (the rest of it takes about a third more)")
     ap.add_argument("--forms", default="plain_fp32,plain_bf16,learned_bf16",
                     help="comma list; the first is the headline `value`")
     ap.add_argument("--learned-warmup-steps", type=int, default=2000,
@@ -63,11 +72,31 @@ def parse():
                          "the train step's MLP stages")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--quality-steps", type=int, default=2000,
+                    help="train steps of plain FP32 and learned BF16 on identical batches before their "
+                         "PSNR against the float64 ground truth is measured (0: no quality run)")
+    ap.add_argument("--quality-views", type=int, default=2, help="held-out 800x800 views rendered for PSNR")
+    ap.add_argument("--quality-average-last", type=int, default=500,
+                    help="also evaluate the weights averaged over the last N train steps")
     return ap.parse_args()
+
+
+def under_profiler() -> bool:
+    """rocprofv3 starts the program with its tool library named in ROCP_TOOL_LIBRARIES
+    and its settings in ROCPROF_* variables."""
+    return any(k.startswith("ROCPROF") or k == "ROCP_TOOL_LIBRARIES" for k in os.environ)
 
 
 def main():
     args = parse()
+    if not args.no_graphs and under_profiler():
+        # A kernel trace of the learned form with HIP-graph replays ended in a segmentation
+        # fault of the profiled process, and a counter run of it stopped printing; the same
+        # form ran eagerly under the same profiler without either.  Under a profiler the
+        # train steps therefore run eagerly.
+        print("bench: running under rocprofv3: train steps run eagerly (no HIP graph capture)",
+              file=sys.stderr, flush=True)
+        args.no_graphs = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU; "
               "the path shards by ray ranges but is not run on several GPUs.",
@@ -121,9 +150,13 @@ def main():
     torch.cuda.synchronize()
 
     forms = [f for f in args.forms.split(",") if f]
+    quality = None
+    if args.quality_steps > 0:
+        quality = quality_data(nerf, scene, args, dev)
     results = {}
     for form in forms:
-        results[form] = measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches)
+        q = quality if form in ("plain_fp32", "learned_bf16") else None
+        results[form] = measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches, q)
     head = results[forms[0]]
     hbm = None if args.no_hbm_kernels else measure_hbm_kernels(nerf, dev)
     stages = None if args.no_hbm_kernels else measure_train_stages(nerf, dev)
@@ -161,22 +194,79 @@ def main():
         "recompute": recompute,
     }
     if "learned_bf16" in results and "plain_fp32" in results and results["plain_fp32"]["train_rays_per_s"]:
+        lp, pp = results["learned_bf16"], results["plain_fp32"]
         line["learned_vs_plain"] = {
-            "render_speedup": round(results["learned_bf16"]["render_rays_per_s"] /
-                                    results["plain_fp32"]["render_rays_per_s"], 2),
-            "train_speedup": round(results["learned_bf16"]["train_rays_per_s"] /
-                                   results["plain_fp32"]["train_rays_per_s"], 2),
-            # not measured by this run: the 2000-step training comparison is its own driver
-            "psnr": {"source": "nerf-tf2_amd/quality.py, 6 seeds x 2000 steps, weight-averaged; "
-                               "profiles/quality_blob_2000steps.md",
-                     "learned_minus_plain_fp32_db": -0.055, "control_redraw_db": -0.051}}
+            "render_speedup": round(lp["render_rays_per_s"] / pp["render_rays_per_s"], 2),
+            "train_speedup": round(lp["train_rays_per_s"] / pp["train_rays_per_s"], 2),
+        }
+        if "quality" in lp and "quality" in pp:
+            ql, qp = lp["quality"], pp["quality"]
+            line["learned_vs_plain"]["psnr"] = {
+                "measured": f"this run: {args.quality_steps} train steps each on identical 4096-ray batches, "
+                            f"{args.quality_views} held-out 800x800 views vs the float64 ground truth, seed 0",
+                "plain_fp32_db": qp["final"]["psnr_mean"], "learned_bf16_db": ql["final"]["psnr_mean"],
+                "learned_minus_plain_fp32_db": round(ql["final"]["psnr_mean"] - qp["final"]["psnr_mean"], 3),
+                "max_abs_rgb_err_plain_fp32": qp["final"]["max_abs_rgb_err"],
+                "max_abs_rgb_err_learned_bf16": ql["final"]["max_abs_rgb_err"],
+                "max_abs_rgb_err_learned_vs_plain": max(ql["final"]["max_abs_rgb_err_vs_plain_fp32"]),
+            }
+            if "averaged" in ql and "averaged" in qp:
+                line["learned_vs_plain"]["psnr"].update({
+                    "averaged_last": args.quality_average_last,
+                    "plain_fp32_avg_db": qp["averaged"]["psnr_mean"],
+                    "learned_bf16_avg_db": ql["averaged"]["psnr_mean"],
+                    "learned_minus_plain_fp32_avg_db": round(ql["averaged"]["psnr_mean"] -
+                                                             qp["averaged"]["psnr_mean"], 3)})
     print(json.dumps(line))
 
 
-def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
+def quality_data(nerf, scene, args, dev):
+    """The quality run's data: args.quality_steps distinct 4096-ray batches drawn
+    over 100 seed-placed views with float64 ground truth, and args.quality_views
+    held-out 800x800 views with theirs (as nerf-tf2_amd/quality.py)."""
+    import torch
+    quality = importlib.import_module("nerf-tf2_amd.quality")
+    sc = nerf.scene
+    t0 = time.perf_counter()
+    qb = quality.make_batches(nerf, scene, args.quality_steps, device=dev)
+    K = sc.intrinsics(800, 800, 1111.1)
+    views = []
+    for pose in sc.random_poses(args.quality_views, 999):
+        o, d = nerf.ray_utils.get_rays(800, 800, K, pose, device=dev)
+        gt, _, _ = scene.render(o.double(), d.double(), n_samples=1024, chunk=4096)
+        views.append((o, d, gt))
+    torch.cuda.synchronize()
+    print(f"bench: quality data ({args.quality_steps} batches, {args.quality_views} views) "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    return {"batches": qb, "views": views, "renders": {}}
+
+
+def evaluate_quality(net, learned, views, ref=None):
+    """PSNR (per element, float64) and max |rgb error| of each held-out view
+    against its float64 ground truth (and against `ref`'s renders)."""
+    import torch
+    sc = importlib.import_module("nerf-tf2_amd").scene
+    preds = []
+    for o, d, _ in views:
+        n = o.shape[0]
+        nr = torch.full((n, 1), sc.NEAR, device=o.device)
+        fr = torch.full((n, 1), sc.FAR, device=o.device)
+        out = net.predict(o, d, nr, fr, fallback=True) if learned else net.predict(o, d, nr, fr)
+        preds.append(out["pred_rgb"])
+    rec = {"psnr": [round(sc.psnr(v[2], p), 3) for p, v in zip(preds, views)],
+           "max_abs_rgb_err": max(round(float((p.double() - v[2]).abs().max()), 4) for p, v in zip(preds, views))}
+    rec["psnr_mean"] = round(sum(rec["psnr"]) / len(rec["psnr"]), 3)
+    if ref is not None:
+        rec["max_abs_rgb_err_vs_plain_fp32"] = [round(float((a - b).abs().max()), 4) for a, b in zip(preds, ref)]
+    return rec, preds
+
+
+def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches, quality=None):
     """Train (timed, graph-replayed 4096-ray steps) and render (timed, K steps of
-    one 800x800 view) rays/s of one form.  The learned form trains its warm-up
-    steps first so that its render runs with a trained proposal."""
+    one 800x800 view) rays/s of one form.  With `quality`, the form is trained on
+    the quality run's batches (timing every step) and its held-out views are
+    evaluated before the render is timed; otherwise the learned form trains its
+    warm-up steps first so that its render runs with a trained proposal."""
     import numpy as np
     import torch
     n_rays = ro.shape[0]
@@ -185,7 +275,9 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
     net = nerf.ProposalNeRF(P, prec, seed=0, device=dev) if learned else nerf.NeRF(P, prec, seed=0, device=dev)
     out = {"train_rays_per_s": None, "train_ms_per_step": None}
     nb = len(batches)
-    if args.train_steps > 0 or learned:
+    if quality is not None:
+        out.update(train_for_quality(form, net, learned, args, nerf, dev, quality))
+    elif args.train_steps > 0 or learned:
         step = net.train_step if args.no_graphs else nerf.GraphedTrainStep(net, batches[0][0].shape[0])
         pre = args.learned_warmup_steps if learned else 2
         for i in range(pre):
@@ -230,7 +322,7 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
     out["render_ms_per_step"] = round(1e3 * t_render / args.steps, 3)
     if learned:
         out["fallback_fraction"] = round(fb, 4)
-        out["warmup_train_steps"] = args.learned_warmup_steps
+        out["warmup_train_steps"] = args.quality_steps if quality is not None else args.learned_warmup_steps
     full = [e0.elapsed_time(e1) for e0, e1, m in events if m == chunk * 192]
     if full:
         avg_ms = float(np.mean(full))
@@ -240,6 +332,63 @@ def measure_form(form, args, nerf, P, dev, ro, rd, near, far, chunk, batches):
         out["mlp_tflops"] = round(flop / (avg_ms * 1e-3) / 1e12, 2)
     del net
     torch.cuda.empty_cache()
+    return out
+
+
+def train_for_quality(form, net, learned, args, nerf, dev, quality):
+    """args.quality_steps graph-replayed train steps on the quality batches (all
+    timed), then the held-out views with the final weights and with the weights
+    averaged over the last args.quality_average_last steps (the final weights are
+    restored afterwards)."""
+    import torch
+    sc = nerf.scene
+    qb = quality["batches"]
+    n = qb[0][0].shape[0]
+    near = torch.full((n, 1), sc.NEAR, device=dev)
+    far = torch.full((n, 1), sc.FAR, device=dev)
+    torch.manual_seed(1000)                      # the samplers' draws: the same stream for every form
+    step = net.train_step if args.no_graphs else nerf.GraphedTrainStep(net, n)
+    mods = net.optimizer.models
+    avg_n = min(args.quality_average_last, len(qb))
+    sums = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (o, d, gt) in enumerate(qb):
+        step(o, d, near, far, gt)
+        if avg_n and i >= len(qb) - avg_n:
+            if sums is None:
+                sums = [m.params.detach().clone() for m in mods]
+            else:
+                for acc, m in zip(sums, mods):
+                    acc.add_(m.params.detach())
+        if (i + 1) % 500 == 0:
+            print(f"bench: {form} train step {i + 1}/{len(qb)}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del step
+    out = {"train_rays_per_s": round(n * len(qb) / t_train, 1), "train_ms_per_step": round(1e3 * t_train / len(qb), 3),
+           "train_steps_timed": len(qb)}
+    ref = quality["renders"].get("plain_fp32")
+    q = {}
+    with torch.no_grad():
+        q["final"], preds = evaluate_quality(net, learned, quality["views"], ref)
+        if learned:
+            q["final"]["fallback_fraction"] = round(net.last_fallback_fraction, 4)
+        if form == "plain_fp32":
+            quality["renders"]["plain_fp32"] = preds
+        if sums is not None:
+            final = [m.params.detach().clone() for m in mods]
+            for acc, m in zip(sums, mods):
+                m.params.copy_(acc / float(avg_n))
+                m.repack()
+            q["averaged"], _ = evaluate_quality(net, learned, quality["views"], None)
+            if learned:
+                q["averaged"]["fallback_fraction"] = round(net.last_fallback_fraction, 4)
+            for f, m in zip(final, mods):
+                m.params.copy_(f)
+                m.repack()
+    print(f"bench: {form} quality {json.dumps(q)}", file=sys.stderr, flush=True)
+    out["quality"] = q
     return out
 
 
@@ -418,7 +567,11 @@ def _render_chunk(net, nerf, ro, rd, near, far, timed_kernel, events):
 
 def cpu_baseline(seconds: float):
     """The oracle (PyTorch-CPU float32 restatement of the reference path) on the
-    host cores: forward of a bounded ray sample of the same render workload."""
+    host cores, on the workloads of the measurement plan: config 0 in full (a
+    1024-ray forward and one Adam train step on a 100x100 view), one 32,768-ray
+    chunk of the 800x800 render (a whole view is 19.53 such chunks) and one
+    4096-ray train step.  `seconds` bounds the render sample: the chunk runs in
+    4096-ray slices until it is done or `seconds` have passed."""
     import numpy as np
     import torch
     oracle = importlib.import_module("oracle.restatement")
@@ -428,43 +581,49 @@ def cpu_baseline(seconds: float):
     spec = oracle.MLPSpec()
     wc = [torch.from_numpy(w) for w in oracle.glorot_weights(spec, 0)]
     wf = [torch.from_numpy(w) for w in oracle.glorot_weights(spec, 1)]
-    K = sc.intrinsics(800, 800, 1111.1)
-    ro, rd = oracle.get_rays(800, 800, K, sc.random_poses(1, 1234)[0])
     P = oracle.make_params()
-    n = 512
-    done = 0
+    g = torch.Generator().manual_seed(0)
+
+    def rays(H, W, focal, n, seed):
+        ro, rd = oracle.get_rays(H, W, sc.intrinsics(H, W, focal), sc.random_poses(1, seed)[0])
+        idx = np.random.default_rng(seed).integers(0, H * W, size=n)
+        f = lambda a: torch.from_numpy(np.ascontiguousarray(a[idx], dtype=np.float32))
+        return f(ro), f(rd), torch.full((n, 1), 2.0), torch.full((n, 1), 6.0)
+
+    def forward(o, d, nr, fr):
+        n = o.shape[0]
+        with torch.no_grad():
+            oracle.forward(P, spec, wc, wf, o, d, nr, fr, torch.rand(n, 64, generator=g), torch.rand(n, 128, generator=g))
+
+    def train_step(o, d, nr, fr, ws):
+        n = o.shape[0]
+        _, _, gc, gf, _ = oracle.train_step_grads(P, spec, ws[0], ws[1], o, d, nr, fr, torch.rand(n, 3, generator=g),
+                                                  torch.rand(n, 64, generator=g), torch.rand(n, 128, generator=g))
+        new = oracle.AdamTF(ws[0] + ws[1]).step(ws[0] + ws[1], gc + gf)
+        return [new[:len(ws[0])], new[len(ws[0]):]]
+
+    out = {"unit": "rays/s", "cores": threads, "kind": "port"}
+    # config 0: 1024 rays of a 100x100 view, forward and one train step
+    c0 = rays(100, 100, 138.9, 1024, 3)
     t0 = time.perf_counter()
-    with torch.no_grad():
-        while True:
-            sl = slice(done % 600000, done % 600000 + n)
-            o = torch.from_numpy(np.ascontiguousarray(ro[sl], dtype=np.float32))
-            d = torch.from_numpy(np.ascontiguousarray(rd[sl], dtype=np.float32))
-            oracle.forward(P, spec, wc, wf, o, d, torch.full((n, 1), 2.0), torch.full((n, 1), 6.0),
-                           torch.rand(n, 64), torch.rand(n, 128))
-            done += n
-            if time.perf_counter() - t0 > seconds:
-                break
-    dt = time.perf_counter() - t0
-    # train steps of the oracle (forward, autograd backward, TF-form Adam) on 1024-ray
-    # batches of the same view (the config-0 batch size), for about a third of the time
-    wc_t = [w.clone() for w in wc]
-    wf_t = [w.clone() for w in wf]
-    opt = oracle.AdamTF(wc_t + wf_t)
-    nt, steps = 1024, 0
-    t1 = time.perf_counter()
-    while True:
-        sl = slice((steps * nt) % 600000, (steps * nt) % 600000 + nt)
-        o = torch.from_numpy(np.ascontiguousarray(ro[sl], dtype=np.float32))
-        d = torch.from_numpy(np.ascontiguousarray(rd[sl], dtype=np.float32))
-        _, _, gc, gf, _ = oracle.train_step_grads(P, spec, wc_t, wf_t, o, d, torch.full((nt, 1), 2.0),
-                                                  torch.full((nt, 1), 6.0), torch.rand(nt, 3), torch.rand(nt, 64),
-                                                  torch.rand(nt, 128))
-        new_w = opt.step(wc_t + wf_t, gc + gf)
-        wc_t, wf_t = new_w[:len(wc_t)], new_w[len(wc_t):]
-        steps += 1
-        if time.perf_counter() - t1 > seconds / 3:
-            break
-    dt_train = time.perf_counter() - t1
+    forward(*c0)
+    t_c0f = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    train_step(*c0, [[w.clone() for w in wc], [w.clone() for w in wf]])
+    t_c0t = time.perf_counter() - t0
+    # one 32,768-ray chunk of the 800x800 render (rays of the bench's view)
+    n_chunk = 32768
+    r1 = rays(800, 800, 1111.1, n_chunk, 1234)
+    n_run = 0
+    t0 = time.perf_counter()
+    while n_run < n_chunk and (n_run == 0 or time.perf_counter() - t0 < seconds):
+        forward(*(x[n_run:n_run + 4096] for x in r1))
+        n_run += 4096
+    t_chunk = time.perf_counter() - t0
+    # one 4096-ray train step on the 800x800 view's rays
+    t0 = time.perf_counter()
+    train_step(*(x[:4096] for x in r1), [[w.clone() for w in wc], [w.clone() for w in wf]])
+    t_train = time.perf_counter() - t0
     cpu_model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -473,12 +632,16 @@ def cpu_baseline(seconds: float):
                 break
     except OSError:
         pass
-    return {"value": round(done / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"{done} rays of the 800x800 render workload (coarse+fine forward, {n}-ray chunks, "
-                      f"{dt:.1f} s, torch-CPU float32 restatement, {cpu_model})",
-            "train_rays_per_s": round(steps * nt / dt_train, 1),
-            "train_sample": f"{steps} train steps of {nt} rays (forward, autograd backward, Adam), {dt_train:.1f} s"}
-
-
-if __name__ == "__main__":
-    main()
+    render_rps = n_run / t_chunk
+    out.update({
+        "value": round(render_rps, 1),
+        "sample": (f"{n_run} rays of one 32,768-ray chunk of the 800x800 render (coarse+fine forward in 4096-ray "
+                   f"slices, {t_chunk:.1f} s; a whole view = 19.53 chunks = {640000 / render_rps:.0f} s at this "
+                   f"rate), torch-CPU float32 restatement of the reference path, {threads} threads, {cpu_model}"),
+        "train_rays_per_s": round(4096 / t_train, 1),
+        "train_sample": f"one 4096-ray train step (forward, autograd backward, Adam), {t_train:.1f} s",
+        "config0": {"forward_s": round(t_c0f, 2), "train_step_s": round(t_c0t, 2),
+                    "forward_rays_per_s": round(1024 / t_c0f, 1), "train_rays_per_s": round(1024 / t_c0t, 1),
+                    "workload": "1024 rays of a 100x100 view: one forward, one Adam train step"},
+    })
+    return out

@@ -51,11 +51,11 @@ class MLP64:
             h = self._dense(h, extra, Wt[2 * i], Wt[2 * i + 1], 1)
             self.hs.append(h)
             extra = self.ex if i == SKIP else None
-        p = 2 * D
-        self.sigma = self._dense(h, None, Wt[p], Wt[p + 1], 1)
-        self.bott = self._dense(h, None, Wt[p + 2], Wt[p + 3], 0)
-        self.h9 = self._dense(self.bott, self.ed, Wt[p + 4], Wt[p + 5], 1)
-        self.rgb = self._dense(self.h9, None, Wt[p + 6], Wt[p + 7], 2)
+        p = 2 * D   # Keras order after the trunk: bottleneck, rgb branch, rgb head, sigma head
+        self.bott = self._dense(h, None, Wt[p], Wt[p + 1], 0)
+        self.h9 = self._dense(self.bott, self.ed, Wt[p + 2], Wt[p + 3], 1)
+        self.rgb = self._dense(self.h9, None, Wt[p + 4], Wt[p + 5], 2)
+        self.sigma = self._dense(h, None, Wt[p + 6], Wt[p + 7], 1)
         return self.rgb, self.sigma
 
     def _dx(self, dZ, k, K, mask_from):
@@ -75,14 +75,14 @@ class MLP64:
         p = 2 * D
         dz_rgb = g_rgb * self.rgb * (1.0 - self.rgb)                 # sigmoid'
         dz_sig = g_sigma * (self.sigma > 0).double()                 # relu'
-        self._wgrad(self.h9, dz_rgb, G[p + 6], G[p + 7])
-        dz9 = self._dx(dz_rgb, Wt[p + 6], W // 2, self.h9)
-        self._wgrad(self.bott, dz9, G[p + 4], G[p + 5])
-        self._wgrad(self.ed, dz9, G[p + 4], None, row0=W)
-        dz8 = self._dx(dz9, Wt[p + 4], W, None)                      # linear bottleneck
-        self._wgrad(self.hs[-1], dz8, G[p + 2], G[p + 3])
-        self._wgrad(self.hs[-1], dz_sig, G[p], G[p + 1])
-        dh = self._dx(dz8, Wt[p + 2], W, None) + self._dx(dz_sig, Wt[p], W, None)
+        self._wgrad(self.h9, dz_rgb, G[p + 4], G[p + 5])
+        dz9 = self._dx(dz_rgb, Wt[p + 4], W // 2, self.h9)
+        self._wgrad(self.bott, dz9, G[p + 2], G[p + 3])
+        self._wgrad(self.ed, dz9, G[p + 2], None, row0=W)
+        dz8 = self._dx(dz9, Wt[p + 2], W, None)                      # linear bottleneck
+        self._wgrad(self.hs[-1], dz8, G[p], G[p + 1])
+        self._wgrad(self.hs[-1], dz_sig, G[p + 6], G[p + 7])
+        dh = self._dx(dz8, Wt[p], W, None) + self._dx(dz_sig, Wt[p + 6], W, None)
         dz = dh * (self.hs[-1] > 0).double()
         for i in range(D - 1, -1, -1):
             X1, X2 = self.inputs[i]
@@ -107,7 +107,7 @@ def _setup(nerf, oracle):
     Wn = [torch.from_numpy(w).double() for w in oracle.glorot_weights(spec, 11)]
     for i in range(1, len(Wn), 2):
         Wn[i] = torch.randn_like(Wn[i]) * 0.1
-    Wn[2 * D + 1] = Wn[2 * D + 1] + 2.0      # sigma bias: keep densities positive
+    Wn[2 * D + 7] = Wn[2 * D + 7] + 2.0      # sigma bias (last in Keras order): keep densities positive
     target = torch.rand(n, 3, dtype=torch.float64)
     return (t.to(DEV), xyz.contiguous().to(DEV), dirs.contiguous().to(DEV), [w.contiguous().to(DEV) for w in Wn],
             target.to(DEV))
