@@ -645,3 +645,7 @@ def cpu_baseline(seconds: float):
                     "workload": "1024 rays of a 100x100 view: one forward, one Adam train step"},
     })
     return out
+
+
+if __name__ == "__main__":
+    main()
