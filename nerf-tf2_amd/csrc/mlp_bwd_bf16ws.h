@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   static_assert(W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D,
                 "weight-stationary backward is for the 8x256 NeRF MLP");
   static_assert(C::NBR == 4, "rgb branch of 128 features");
+  static_assert(D % 4 == 0, "ReLU bit words are read four layers at a time");
   __shared__ bf16x8 s_act[2][kBlocks * kSlab];  // 2 x 64 KB
   __shared__ float s_dzs[kBlocks * 32];          // dz_sigma of the workgroup's samples
   const int lane = threadIdx.x & 63, j = lane & 31, g = lane >> 5;
@@ -78,7 +79,8 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   // per-lane byte offsets in the rows of sample block sb: the lane's dz row chunk
   // of each 16-feature group (+ 8 g features), its stash mask word, the row start
   auto dz_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * zps + 8 * g) * 2); };
-  auto mask_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps + 2 * g) * 2); };  // this lane's mask word
+  // this lane's mask words: the D words of (wave, lane half) are consecutive in the row
+  auto mask_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps) * 2 + 4 * g * D); };
   auto row0 = [&](int sb) { return (uint32_t)((sb * 32 + j) * sps * 2); };
 
   bf16x8 wlo[kHalf][2], whi[kHalf][2];
@@ -176,20 +178,33 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   }
 
   // ---- bottleneck and trunk, unrolled: step s = 0 is W_D (input dz_bott), then W_{D-1} .. W_1.
+  // The forward's ReLU bit words of trunk layers 4q .. 4q+3 are 16 contiguous bytes
+  // per lane (the mask region's [wave][lane half][layer] layout): loaded once per
+  // sample block every four steps.
+  u32x4 mq[kBlocks];
   static_for<0, D>([&](auto Sc) {
     constexpr int s = decltype(Sc)::value;
     constexpr int layer = D - s;          // packed layer whose transpose is applied
     constexpr int cur = (s + 1) & 1;      // buffer holding this step's input dz
     lds_b8* const in_l = lds_lane(s_act[cur], 0, lane);
     lds_b8* const out_l = lds_lane(s_act[cur ^ 1], 2 * nb0 * 64, lane);  // unused for layer 1
-    // the forward's ReLU bit words of this step's output for the four sample blocks,
-    // requested before the step's weight loads and dz stores: vmcnt retires vector
-    // memory operations in issue order, so a load issued behind them would also wait
-    // for them
+    // the forward's ReLU bit words of this step's output (trunk layer layer - 1) for
+    // the four sample blocks, requested before the step's weight loads and dz stores:
+    // vmcnt retires vector memory operations in issue order, so a load issued behind
+    // them would also wait for them
+    constexpr int ml = layer - 1;
+    if constexpr ((ml & 3) == 3) {
+#pragma unroll
+      for (int sb = 0; sb < kBlocks; ++sb)
+        mq[sb] = __builtin_amdgcn_raw_buffer_load_b128(rst, mask_row(sb),
+                                                       (int)((P.st_mask + 4 * wv * D + 2 * (ml & ~3)) * 2), 0);
+    }
     uint32_t mbits[kBlocks];
 #pragma unroll
-    for (int sb = 0; sb < kBlocks; ++sb)
-      mbits[sb] = ld_u32(rst, mask_row(sb), (P.st_mask + 16 * (layer - 1) + 4 * wv) * 2);
+    for (int sb = 0; sb < kBlocks; ++sb) {
+      const u32x4 q = mq[sb];   // copied out first: an element subscript of a captured vector array
+      mbits[sb] = q[ml & 3];
+    }
     if constexpr (s == 0) {
       // accumulators start from the sigma-head term W_sigma dz_sigma
       const uint32_t hs_off = 4u * (uint32_t)(g * C::KH);

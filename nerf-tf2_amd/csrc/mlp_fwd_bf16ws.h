@@ -18,7 +18,8 @@
 // accumulator layout), so the epilogue of a sample block is: bf16 conversion,
 // ReLU as a packed integer max on the bf16 bits (equal to ReLU before the
 // rounding, which keeps the sign), two LDS stores and, in training, the stash
-// rows.  Each block's epilogue is issued right after its last MFMA, so it
+// rows (the ReLU bits stay in registers and leave as one 16-byte store per
+// sample block every four layers).  Each block's epilogue is issued right after its last MFMA, so it
 // overlaps the MFMAs of the next block.  Out-of-range samples of the last
 // workgroup compute the values of the last sample; their stash stores go
 // through a buffer resource over the workgroup's rows and fall outside its
@@ -193,6 +194,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
   using C = NetC<W, D, SKIP, LX, LD>;
   static_assert(W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D,
                 "weight-stationary kernel is for the 8x256 NeRF MLP");
+  static_assert(D % 4 == 0, "ReLU bit words are written four layers at a time");
   __shared__ bf16x8 s_act[2][kBlocks * kSlab];          // 2 x 64 KB
   __shared__ bf16x8 s_ex[kBlocks * kTX * 64];           // 16 KB
   __shared__ bf16x8 s_ed[kBlocks * kTD * 64];           // 8 KB
@@ -265,11 +267,25 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
     }
   }
 
+  // ReLU bit words of the trunk layers, kept for four layers and then written as
+  // one 16-byte chunk per sample block: the mask region of a stash row holds, per
+  // wave and lane half, D consecutive words (one per trunk layer), so this lane's
+  // words for layers 4q .. 4q+3 are 16 contiguous bytes.
+  uint32_t mbits[kBlocks][4];
+  auto store_masks = [&](int q) {
+#pragma unroll
+    for (int sb = 0; sb < kBlocks; ++sb) {
+      const u32x4 v = {mbits[sb][0], mbits[sb][1], mbits[sb][2], mbits[sb][3]};
+      __builtin_amdgcn_raw_buffer_store_b128(
+          v, rst, row_off(sb) + (uint32_t)((P.st_mask + 2 * ((2 * wv + g) * D + 4 * q)) * 2), 0, 0);
+    }
+  };
+
   // Epilogue of sample block sb of a 256-wide layer: bf16, ReLU, LDS (B layout), stash.
   // out: per-lane pointer at k-step 2 nb0 of sample block 0 of the output buffer
-  // MASK: also the ReLU bit mask of the wave's 64 features (bits of output
-  // block nb0 + i at 8 i + {0..7, 16..23}) at stash offset mask_off
-  auto epilogue = [&](lds_b8* out, bool relu, int64_t st_off, auto mask_c, int64_t mask_off, int sb) {
+  // MASK: also the ReLU bit word of the wave's 64 features (bits of output block
+  // nb0 + i at 8 i + {0..7, 16..23}) of trunk layer `layer`, into mbits
+  auto epilogue = [&](lds_b8* out, bool relu, int64_t st_off, auto mask_c, int layer, int sb) {
     constexpr bool MASK = decltype(mask_c)::value;
     uint32_t bits = 0;
 #pragma unroll
@@ -289,8 +305,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
         bits |= nz_bits(lo, hi) << (8 * i);
       }
     }
-    if constexpr (STASH && MASK)
-      __builtin_amdgcn_raw_buffer_store_b32(bits, rst, row_off(sb) + (uint32_t)((mask_off + 4 * wv + 2 * g) * 2), 0, 0);
+    if constexpr (STASH && MASK) mbits[sb][layer & 3] = bits;
   };
 
   __syncthreads();  // encodings visible
@@ -298,7 +313,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
   lds_b8* const ex_l = lds_lane(s_ex, 0, lane);
   {
     lds_b8* const out_l = lds_lane(s_act[0], 2 * nb0 * 64, lane);
-    run_steps<kT, kTX, true>(acc, whi, cb, nullptr, ex_l, [&](int sb) { epilogue(out_l, true, P.st_h[0], std::true_type{}, P.st_mask, sb); });
+    run_steps<kT, kTX, true>(acc, whi, cb, nullptr, ex_l, [&](int sb) { epilogue(out_l, true, P.st_h[0], std::true_type{}, 0, sb); });
   }
   load_half(whi, 1, SKIP == 0, true);  // layer 1's second half and bias
   cb[0] = bias_acc(rs, P.p_bias[1], nb0, g_off);
@@ -348,7 +363,8 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16ws_kernel(
       cb[0] = bias_acc(rs, P.p_bias[D + 1], wv, g_off);
     }
     const int64_t st_off = l < D ? P.st_h[l] : P.st_bott;
-    run_steps<H, H, false>(acc, whi, cb, in_l, ex_l, [&](int sb) { epilogue(out_l, l < D, st_off, std::integral_constant<bool, (l < D)>{}, P.st_mask + 16 * l, sb); });
+    run_steps<H, H, false>(acc, whi, cb, in_l, ex_l, [&](int sb) { epilogue(out_l, l < D, st_off, std::integral_constant<bool, (l < D)>{}, l, sb); });
+    if constexpr (STASH && l < D && (l & 3) == 3) store_masks(l >> 2);
     if constexpr (l < D) load_half(whi, l + 1, l + 1 == SKIP + 1, true);  // whi is free
     __syncthreads();  // layer l outputs complete
   });
