@@ -214,6 +214,23 @@ int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int6
                               int32_t accumulate, void* workspace, int64_t workspace_bytes,
                               void* stream);
 
+/* nerf_mlp_weight_grads without a workspace and without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The partial tiles live in a static device array of the library,
+ * NERF_WGRAD_STATIC_SCRATCH_BYTES on every GPU it runs on, held until the process
+ * ends; its address is looked up once per device on the first call there.  A call
+ * whose partial tiles need more is rejected with NERF_ERR_INVALID_ARG before
+ * anything is launched (the 8x256 reference net fits every batch up to 10^7
+ * samples in either precision).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams or threads) race.  Nothing in the
+ * array needs clearing (each call writes every partial it reads), and the result
+ * is bitwise that of nerf_mlp_weight_grads (same launches, same order). */
+#define NERF_WGRAD_STATIC_SCRATCH_BYTES (256LL * (256 * 256 + 256) * 4) /* 64.25 MiB */
+int32_t nerf_mlp_weight_grads_noalloc(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
+                                      const void* stash, const void* dz, float* grad_flat,
+                                      int32_t accumulate, void* stream);
+
 /* ------------------------------------------------------------------------
  * Losses and the optimizer step (model.py:32, 148-171, 413-418).
  * ---------------------------------------------------------------------- */

@@ -57,6 +57,7 @@ _SIGS = {
     "nerf_mlp_backward": (_i32, [_descp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nerf_mlp_wgrad_workspace_bytes": (_i64, [_descp, _i64]),
     "nerf_mlp_weight_grads": (_i32, [_descp, _i32, _i64, _vp, _vp, _vp, _i32, _vp, _i64, _vp]),
+    "nerf_mlp_weight_grads_noalloc": (_i32, [_descp, _i32, _i64, _vp, _vp, _vp, _i32, _vp]),
     "nerf_mse_grad": (_i32, [_i64, _vp, _vp, _f32, _vp, _vp, _vp]),
     "nerf_adam_tf": (_i32, [_i64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _vp, _vp, _vp]),
     "nerf_check_finite": (_i32, [_i64, _vp, _vp, _vp]),

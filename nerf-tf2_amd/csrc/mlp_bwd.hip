@@ -10,6 +10,8 @@
 //   Every pre-activation gradient dz_l is written row-major for the weight GEMMs.
 // nerf_mlp_weight_grads: dW_l = X_l^T dz_l and db_l = sum dz_l over the samples,
 //   as a split-K MFMA GEMM whose partial tiles are summed in a fixed order.
+#include <mutex>
+
 #include "common.h"
 #include "mlp_kernels.h"
 #include "mlp_bwd_bf16ws.h"
@@ -1111,6 +1113,56 @@ static int32_t wgrad_empty(const MlpPlan& P, float* grad_flat, int32_t accumulat
         grad_flat, P.n_params);
   NERF_LAUNCH_CHECK("nerf_mlp_weight_grads");
   return NERF_OK;
+}
+
+// Partial tiles of nerf_mlp_weight_grads_noalloc: one array per GPU, held until
+// the process ends.  Host code never passes this name to a kernel (on the host
+// it is a stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) float scratch[NERF_WGRAD_STATIC_SCRATCH_BYTES / 4];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// This device's copy of `scratch`, looked up once per device (with the device current).
+static float* scratch_on_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_ptr[dev] = reinterpret_cast<float*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+extern "C" int32_t nerf_mlp_weight_grads_noalloc(const nerf_mlp_desc* desc, int32_t precision,
+                                                 int64_t M, const void* stash, const void* dz,
+                                                 float* grad_flat, int32_t accumulate,
+                                                 void* stream) {
+  MlpPlan P;
+  NERF_CHECK_ARG(nerf_plan_from_desc(desc, precision, &P) == 0,
+                 "nerf_mlp_weight_grads_noalloc: unsupported MLP description or precision");
+  NERF_CHECK_ARG(M >= 0 && grad_flat, "nerf_mlp_weight_grads_noalloc: bad arguments");
+  hipStream_t s = as_stream(stream);
+  if (M == 0) return wgrad_empty(P, grad_flat, accumulate, s);
+  NERF_CHECK_ARG(stash && dz, "nerf_mlp_weight_grads_noalloc: bad arguments");
+  const int64_t need = wgrad_used_floats(P, precision, M);
+  NERF_CHECK_ARG(need >= 0, "nerf_mlp_weight_grads_noalloc: layer too wide for the bf16 weight GEMM");
+  NERF_CHECK_ARG(need * 4 <= (int64_t)NERF_WGRAD_STATIC_SCRATCH_BYTES,
+                 "nerf_mlp_weight_grads_noalloc: M = %lld needs %lld bytes of partial tiles, the "
+                 "static scratch holds %lld; use nerf_mlp_weight_grads with a workspace",
+                 (long long)M, (long long)(need * 4), (long long)NERF_WGRAD_STATIC_SCRATCH_BYTES);
+  float* part = scratch_on_current_device();
+  if (!part) {
+    nerf_set_error("nerf_mlp_weight_grads_noalloc: no device address for the static scratch");
+    return NERF_ERR_LAUNCH;
+  }
+  return wgrad_run(P, precision, M, stash, dz, grad_flat, accumulate, part, s);
 }
 
 extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
