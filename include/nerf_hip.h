@@ -159,8 +159,10 @@ typedef struct nerf_mlp_desc {
 } nerf_mlp_desc;
 
 /* Number of float32 parameters in the flat "Keras" layout: for each layer in
- * trainable-variable order (dense_0..dense_{D-1}, sigma, dense_D, dense_{D+1},
- * rgb) its kernel (fan_in x fan_out, row-major) then its bias. */
+ * Keras' trainable-variable order of the functional model (model.py:389-392:
+ * dense_0 .. dense_{D-1}, dense_D, dense_{D+1}, rgb, sigma; a density-only net:
+ * dense_0 .. dense_{D-1}, sigma) its kernel (fan_in x fan_out, row-major) then
+ * its bias. */
 int64_t nerf_mlp_param_count(const nerf_mlp_desc* desc);
 /* Bytes of the packed device weights for `precision`. */
 int64_t nerf_mlp_packed_bytes(const nerf_mlp_desc* desc, int32_t precision);

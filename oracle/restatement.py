@@ -222,7 +222,8 @@ class MLPSpec:
         return self.W // 2
 
     def layer_shapes(self) -> List[Tuple[str, int, int]]:
-        """(name, fan_in, fan_out) in Keras trainable-variable order."""
+        """(name, fan_in, fan_out) in Keras trainable-variable order (the functional
+        model's layer order, model.py:389-392): dense_0 .. dense_9, rgb, sigma."""
         out = []
         fan = self.enc_xyz
         for i in range(self.D):
