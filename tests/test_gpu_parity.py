@@ -182,15 +182,17 @@ def _oracle_coarse_weights(oracle, n, Nc, seed):
     w = torch.rand((n, Nc), generator=g) ** 4
     w[0] = 0.0
     w[1] = 0.0
-    w[1, 17] = 1.0          # one-hot
+    w[1, 17 % Nc] = 1.0     # one-hot
     w[2] = 1.0 / Nc         # uniform
     return w.float()
 
 
 @pytest.mark.parametrize("lin_inv_depth,Nc,Nf", [(True, 64, 128), (False, 64, 128), (True, 48, 128),
-                                                   (True, 40, 96), (False, 64, 200)])
+                                                   (True, 40, 96), (False, 64, 200), (True, 64, 256),
+                                                   (False, 2, 1)])
 def test_resample_bitexact(nerf, oracle, lin_inv_depth, Nc, Nf):
-    """(64|48, 128): the register/bitonic resampler; other shapes: the general one."""
+    """(64|48, 128): the register/bitonic resampler; other shapes: the general one,
+    including the largest (64 coarse bins, 256 fine samples) and smallest (2, 1)."""
     n = 1029
     ro, rd, near, far = _rays_batch(nerf, n, seed=11)
     P = nerf.Params(N_coarse=Nc, N_fine=Nf, lin_inv_depth=lin_inv_depth)
