@@ -170,7 +170,13 @@ __device__ __forceinline__ void mfma_regs_f32(floatx16 (&acc)[NBo], const floatx
 }
 
 // acc[nb] += A * B over Q groups of 4 k-steps whose B operand is in the wave's
-// LDS slab act[q][lane]; weights are prefetched one group ahead.
+// LDS slab act[q][lane]; weights and B operands are fetched one group ahead.
+// The loop body has no branch: on the last pass the prefetch re-reads the
+// last group (an in-bounds load whose value is unused), so the wait before
+// each MFMA counts only the loads of its own group.  (With a conditional
+// prefetch, the join of the two paths made the compiler wait for the newest
+// weight loads within a few MFMAs of issuing them, and a B operand read just
+// before its first MFMA stalled every group on the LDS latency.)
 template <int NBo, int Q>
 __device__ __forceinline__ void mfma_lds_f32(floatx16 (&acc)[NBo], const floatx4* __restrict__ wq,
                                              const floatx4* act, int lane) {
@@ -178,24 +184,24 @@ __device__ __forceinline__ void mfma_lds_f32(floatx16 (&acc)[NBo], const floatx4
   floatx4 wa[NBo], wb[NBo];
 #pragma unroll
   for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[nb * 64 + lane];
+  floatx4 ba = act[lane];
   for (int q = 0; q < Q; q += 2) {
 #pragma unroll
     for (int nb = 0; nb < NBo; ++nb) wb[nb] = wq[((q + 1) * NBo + nb) * 64 + lane];
-    floatx4 b = act[q * 64 + lane];
+    const floatx4 bb = act[(q + 1) * 64 + lane];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wa[nb][r], b[r], acc[nb]);
+      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wa[nb][r], ba[r], acc[nb]);
     __builtin_amdgcn_sched_barrier(0);
-    if (q + 2 < Q) {
+    const int qn = q + 2 < Q ? q + 2 : Q - 1;
 #pragma unroll
-      for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[((q + 2) * NBo + nb) * 64 + lane];
-    }
-    b = act[(q + 1) * 64 + lane];
+    for (int nb = 0; nb < NBo; ++nb) wa[nb] = wq[(qn * NBo + nb) * 64 + lane];
+    ba = act[qn * 64 + lane];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wb[nb][r], b[r], acc[nb]);
+      for (int nb = 0; nb < NBo; ++nb) acc[nb] = mfma_f32(wb[nb][r], bb[r], acc[nb]);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
