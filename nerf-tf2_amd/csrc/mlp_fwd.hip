@@ -1,6 +1,10 @@
 // Fused positional-encoding + MLP forward (model.py:289-394) on MFMA.
 //
-// One wave owns 32 samples and carries them through every layer:
+// The reference topology (8 x 256 with the rgb branch) runs the register-
+// resident kernel of mlp_fwd_f32r.h in fp32 and the weight-stationary kernel of
+// mlp_fwd_bf16ws.h in bf16.  The fp32 kernel below serves the other topologies
+// (the proposal and finite-difference nets).  One wave owns 32 samples and
+// carries them through every layer:
 //   * the encoding is computed in registers directly in B-operand slots;
 //   * each layer's output accumulates in registers (floatx16 per 32 features),
 //     gets bias + activation, and is written to the wave's LDS slab as the next
@@ -16,6 +20,7 @@
 
 #include "mlp_fwd_bf16.h"
 #include "mlp_fwd_bf16ws.h"
+#include "mlp_fwd_f32r.h"
 
 namespace {
 
@@ -117,11 +122,16 @@ template <int W, int D, int SKIP, int LX, int LD>
 int launch_fwd_f32(const MlpPlan& P, const void* packed, int64_t M, const float* xyz,
                    const float* dirs, const float* ro, const float* rd, const float* tv, int spr,
                    float* rgb, float* sigma, void* stash, hipStream_t s) {
-  const int64_t nblk = (M + 31) / 32;
-  const int64_t grid = (nblk + 3) / 4;
-  mlp_fwd_f32_kernel<W, D, SKIP, LX, LD><<<(unsigned)grid, 256, 0, s>>>(
-      P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma,
-      reinterpret_cast<float*>(stash));
+  if constexpr (W == 256 && LD > 0 && SKIP >= 0 && SKIP + 1 < D) {
+    // the reference topology: the register-resident kernel (16 samples per wave)
+    launch_fwd_f32r<W, D, SKIP, LX, LD>(P, packed, M, xyz, dirs, ro, rd, tv, spr, rgb, sigma, stash, s);
+  } else {
+    const int64_t nblk = (M + 31) / 32;
+    const int64_t grid = (nblk + 3) / 4;
+    mlp_fwd_f32_kernel<W, D, SKIP, LX, LD><<<(unsigned)grid, 256, 0, s>>>(
+        P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma,
+        reinterpret_cast<float*>(stash));
+  }
   return 0;
 }
 

@@ -25,6 +25,18 @@ __device__ int fwd_row_bf16(const MlpPlan& P, int m, int t, int r, int g) {
   return (m == 0 ? 0 : P.W) + ef;
 }
 
+// Input row fed by k-step s (0 .. ke4 + kh4 - 1: the encoding's first) on lane
+// quarter q of MFMA layer m in the register-resident fp32 forward.
+__device__ int fwd_row_f32x4(const MlpPlan& P, int m, int s, int q) {
+  const int ke4 = P.ks_enc[m] == 0 ? 0 : (P.enc_L[m] == P.LX ? P.S4X : P.S4D);
+  if (s < ke4) {
+    const int ef = enc_feat4(q, s, P.enc_L[m]);
+    return ef < 0 ? -1 : (m == 0 ? 0 : P.W) + ef;
+  }
+  const int t = s - ke4;  // hidden k-step 4 nb + r reads G(nb, r, q)
+  return 16 * (t / 4) + 4 * q + t % 4;
+}
+
 __device__ float kernel_at(const MlpPlan& P, const float* flat, int kl, int row, int col) {
   if (row < 0 || row >= P.fan_in[kl] || col < 0 || col >= P.fan_out[kl]) return 0.f;
   return flat[P.k_off[kl] + (int64_t)row * P.fan_out[kl] + col];
@@ -35,11 +47,44 @@ __device__ float kernel_at(const MlpPlan& P, const float* flat, int kl, int row,
 //   y in [n_mfma, 2 n_mfma)     backward (transposed) A operands
 //   y in [2 n_mfma, 3 n_mfma)   biases
 //   y == 3 n_mfma               sigma head, y == 3 n_mfma + 1 rgb head
+//   y in [3 n_mfma + 2, 4 n_mfma + 2)  register-resident fp32 forward A operands
+//   y == 4 n_mfma + 2           both heads in natural feature order (fp32)
 __global__ void pack_kernel(MlpPlan P, const float* __restrict__ flat, char* __restrict__ out) {
   const int seg = blockIdx.y;
   const int nm = P.n_mfma;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+  if (seg >= 3 * nm + 2) {
+    if (P.prec != 0) return;
+    if (seg < 4 * nm + 2) {
+      // e = ((grp * NBo4 + nb) * 64 + lane) * 4 + r; k-step 4 grp + r; lane = i + 16 q
+      const int m = seg - (3 * nm + 2), kl = P.keras_of[m], NBo4 = 2 * P.nb_out[m];
+      const int ke4 = P.ks_enc[m] == 0 ? 0 : (P.enc_L[m] == P.LX ? P.S4X : P.S4D);
+      const int kh4 = P.ks_hid[m] == 0 ? 0 : P.W / 4;
+      float* o = reinterpret_cast<float*>(out + P.p_fwd4[m]);
+      const int64_t n = (int64_t)(ke4 + kh4) * NBo4 * 64;
+      for (int64_t e = tid; e < n; e += nthr) {
+        const int r = (int)(e % 4), lane = (int)((e / 4) % 64);
+        const int nb = (int)((e / 256) % NBo4), grp = (int)(e / (256 * NBo4));
+        const int i = lane & 15, q = lane >> 4;
+        o[e] = kernel_at(P, flat, kl, fwd_row_f32x4(P, m, 4 * grp + r, q), 16 * nb + i);
+      }
+      return;
+    }
+    float* hs = reinterpret_cast<float*>(out + P.p_hs_nat);
+    for (int64_t e = tid; e < P.W + 4; e += nthr)
+      hs[e] = e < P.W ? kernel_at(P, flat, P.kl_sigma, (int)e, 0) : (e == P.W ? flat[P.b_off[P.kl_sigma]] : 0.f);
+    if (!P.rgb) return;
+    float* hr = reinterpret_cast<float*>(out + P.p_hr_nat);
+    for (int64_t e = tid; e < (int64_t)P.WR * 4 + 4; e += nthr) {
+      const int c = (int)(e % 4);
+      if (e < (int64_t)P.WR * 4)
+        hr[e] = c < 3 ? kernel_at(P, flat, P.kl_rgb, (int)(e / 4), c) : 0.f;
+      else
+        hr[e] = c < 3 ? flat[P.b_off[P.kl_rgb] + c] : 0.f;
+    }
+    return;
+  }
   if (seg < nm) {
     const int m = seg, kl = P.keras_of[m], NBo = P.nb_out[m];
     if (P.prec == 0) {
@@ -161,7 +206,7 @@ extern "C" int32_t nerf_mlp_pack(const nerf_mlp_desc* desc, int32_t precision, c
                                  void* packed, void* stream) {
   NERF_PLAN_OR_FAIL(desc, precision, P, "nerf_mlp_pack");
   NERF_CHECK_ARG(flat && packed, "nerf_mlp_pack: null pointer");
-  dim3 grid(64, 3 * P.n_mfma + 2), block(256);
+  dim3 grid(64, 4 * P.n_mfma + 3), block(256);
   pack_kernel<<<grid, block, 0, as_stream(stream)>>>(P, flat, reinterpret_cast<char*>(packed));
   NERF_LAUNCH_CHECK("nerf_mlp_pack");
   return NERF_OK;

@@ -44,6 +44,40 @@ __host__ __device__ inline int enc_feat(int g, int e, int L) {
 // Hidden feature held by accumulator i of output block nb on lane half g.
 __host__ __device__ inline int cfeat(int nb, int i, int g) { return 32 * nb + 8 * (i / 4) + 4 * g + (i % 4); }
 
+// Register-resident fp32 forward (v_mfma_f32_16x16x4f32): one wave owns 16
+// samples, lane l = j + 16 q (j = sample, q = lane quarter).  Output block nb
+// (16 features) holds in accumulator r of lane (j, q) the feature
+//   G(nb, r, q) = 16 nb + 4 q + r
+// which is also what k-step 4 nb + r of the next layer reads on quarter q, so
+// a layer's output is the next layer's B operand in place.  An encoding of L
+// frequencies takes S4(L) slots per quarter (a multiple of 4): quarter 0 holds
+// x, y, z, pad, then (sin, cos) pairs 0 .. S4/2 - 3; quarter q >= 1 holds
+// pairs S4/2 - 2 + (q - 1) S4/2 + e/2; pairs >= 3L are pads.
+// Inverse of enc_feat: lane-half slot index g * K + e (K slots per half) of
+// reference encoding feature f of an encoding with L frequencies.
+__host__ __device__ inline int enc_slot_of(int f, int L, int K) {
+  if (f < 3) return f;
+  const int P0 = (3 * L - 2) / 2, p = (f - 3) / 2, sc = (f - 3) % 2;
+  return p < P0 ? 4 + 2 * p + sc : K + 2 * (p - P0) + sc;
+}
+
+__host__ __device__ constexpr int enc_slots4(int L) { return ((3 * L + 3) / 2 + 3) / 4 * 4; }
+__host__ __device__ inline int enc_feat4(int q, int e, int L) {
+  const int S = enc_slots4(L), P0 = S / 2 - 2;
+  if (e < 0 || e >= S) return -1;
+  int p;
+  if (q == 0) {
+    if (e < 3) return e;
+    if (e == 3) return -1;
+    p = (e - 4) / 2;
+  } else {
+    p = P0 + (q - 1) * (S / 2) + e / 2;
+  }
+  if (p >= 3 * L) return -1;
+  const int c = p / L, l = p % L;
+  return 3 + c * 2 * L + 2 * l + e % 2;
+}
+
 struct MlpPlan {
   int D, W, skip, LX, LD, rgb;
   int NB, NBR, WR;      // hidden blocks, rgb-branch blocks, rgb-branch width
@@ -67,6 +101,12 @@ struct MlpPlan {
   // packed byte offsets
   int64_t p_fwd[NERF_MAX_LAYERS], p_bwd[NERF_MAX_LAYERS], p_bias[NERF_MAX_LAYERS];
   int64_t p_head_sigma, p_head_rgb, packed_bytes;
+  // fp32 only: A operands of the register-resident forward (per layer: the
+  // encoding's k-step groups first, then the hidden input's; a group is 4
+  // k-steps of every output block of 16), and both heads in natural feature
+  // order (sigma: W weights + bias; rgb: [W/2][4] + bias[4]); -1 for bf16
+  int64_t p_fwd4[NERF_MAX_LAYERS], p_hs_nat, p_hr_nat;
+  int S4X, S4D;  // encoding slots per lane quarter in that forward
   int prec;
   // stash (elements per sample; fp32 or bf16 elements) and dz (floats per sample)
   int64_t st_encx, st_h[NERF_MAX_LAYERS], st_bott, st_rbr, st_encd, stash_per_sample;
@@ -192,6 +232,26 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
   align(256);
   P.p_head_rgb = b;  // [2][16*NBR][4] fp32 + bias[4]
   if (P.rgb) b += (int64_t)2 * 16 * P.NBR * 16 + 16;
+  P.S4X = enc_slots4(LX);
+  P.S4D = LD ? enc_slots4(LD) : 0;
+  P.p_hs_nat = P.p_hr_nat = -1;
+  for (int i = 0; i < P.n_mfma; ++i) {
+    P.p_fwd4[i] = -1;
+    if (prec != 0) continue;
+    align(256);
+    P.p_fwd4[i] = b;
+    const int ke4 = P.ks_enc[i] == 0 ? 0 : (P.enc_L[i] == LX ? P.S4X : P.S4D);
+    const int kh4 = P.ks_hid[i] == 0 ? 0 : W / 4;
+    b += (int64_t)(ke4 + kh4) * (2 * P.nb_out[i]) * 64 * 4;
+  }
+  if (prec == 0) {
+    align(256);
+    P.p_hs_nat = b;
+    b += (int64_t)W * 4 + 16;
+    align(256);
+    P.p_hr_nat = b;
+    if (P.rgb) b += (int64_t)P.WR * 16 + 16;
+  }
   align(256);
   P.packed_bytes = b;
   // stash (elements per sample)
