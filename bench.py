@@ -42,9 +42,9 @@ RENDER_FLOP_PER_RAY = (64 + 192) * MLP_FLOP_PER_SAMPLE
 
 # HBM bytes per fine-model MLP launch (FETCH_SIZE + WRITE_SIZE, each scaled by the
 # same counter on a 1 GiB device copy), from the rocprofv3 --pmc runs whose commands
-# and figures are in profiles/mlp_fwd_f32_render800.md and
+# and figures are in profiles/mlp_fwd_f32r_render800.md and
 # profiles/mlp_fwd_bf16ws_render800.md.
-TRAFFIC_BYTES_PER_LAUNCH = {"fp32": 152.6e6, "bf16": 142.9e6}
+TRAFFIC_BYTES_PER_LAUNCH = {"fp32": 153.8e6, "bf16": 142.9e6}
 
 
 def parse():
