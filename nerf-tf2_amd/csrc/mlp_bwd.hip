@@ -1,7 +1,9 @@
 // Hand-written backward of the fused NeRF MLP (the gradient TF's GradientTape
 // takes through model.py:355-392) and its weight gradients.
 //
-// nerf_mlp_backward: one wave per 32 samples walks the layers top-down.
+// nerf_mlp_backward: one wave per 32 samples walks the layers top-down (the
+// reference topology in fp32 runs the 16-sample register-resident twin of
+// mlp_bwd_f32r.h, same rows in and out).
 //   rgb = sigmoid(z):  dz_rgb = d_rgb * rgb (1 - rgb)
 //   sigma = relu(z):   dz_sigma = d_sigma * [sigma > 0]      (softplus: * (1 - e^-sigma))
 //   dh = W dz (MFMA with the transposed packed weights; the accumulators of one
@@ -15,6 +17,7 @@
 #include "common.h"
 #include "mlp_kernels.h"
 #include "mlp_bwd_bf16ws.h"
+#include "mlp_bwd_f32r.h"
 
 namespace {
 
@@ -855,10 +858,15 @@ template <int W, int D, int SKIP, int LX, int LD>
 int launch_bwd_f32(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,
                    const float* d_sigma, const float* rgb, const float* sigma, const void* stash,
                    void* dz, hipStream_t s) {
-  const int64_t nblk = (M + 31) / 32;
-  mlp_bwd_f32_kernel<W, D, SKIP, LX, LD><<<(unsigned)((nblk + 3) / 4), 256, 0, s>>>(
-      P, reinterpret_cast<const char*>(packed), M, d_rgb, d_sigma, rgb, sigma,
-      reinterpret_cast<const float*>(stash), reinterpret_cast<float*>(dz));
+  if constexpr (W == 256 && LD > 0 && SKIP >= 0 && SKIP + 1 < D) {
+    // the reference topology: the register-resident chain (16 samples per wave)
+    launch_bwd_f32r<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
+  } else {
+    const int64_t nblk = (M + 31) / 32;
+    mlp_bwd_f32_kernel<W, D, SKIP, LX, LD><<<(unsigned)((nblk + 3) / 4), 256, 0, s>>>(
+        P, reinterpret_cast<const char*>(packed), M, d_rgb, d_sigma, rgb, sigma,
+        reinterpret_cast<const float*>(stash), reinterpret_cast<float*>(dz));
+  }
   return 0;
 }
 

@@ -49,11 +49,28 @@ __device__ float kernel_at(const MlpPlan& P, const float* flat, int kl, int row,
 //   y == 3 n_mfma               sigma head, y == 3 n_mfma + 1 rgb head
 //   y in [3 n_mfma + 2, 4 n_mfma + 2)  register-resident fp32 forward A operands
 //   y == 4 n_mfma + 2           both heads in natural feature order (fp32)
+//   y in [4 n_mfma + 3, 5 n_mfma + 3)  register-resident fp32 dX chain A operands
 __global__ void pack_kernel(MlpPlan P, const float* __restrict__ flat, char* __restrict__ out) {
   const int seg = blockIdx.y;
   const int nm = P.n_mfma;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+  if (seg >= 4 * nm + 3) {
+    // e = ((grp * NBi + nb) * 64 + lane) * 4 + r: A[n = input feature 16 nb + i][k = output
+    // feature 16 grp + 4 q + r] of layer m (lane = i + 16 q)
+    const int m = seg - (4 * nm + 3);
+    if (P.prec != 0 || m == 0) return;
+    const int kl = P.keras_of[m], NBi = P.W / 16, G = 2 * P.nb_out[m];
+    float* o = reinterpret_cast<float*>(out + P.p_bwd4[m]);
+    const int64_t n = (int64_t)G * NBi * 64 * 4;
+    for (int64_t e = tid; e < n; e += nthr) {
+      const int r = (int)(e % 4), lane = (int)((e / 4) % 64);
+      const int nb = (int)((e / 256) % NBi), grp = (int)(e / (256 * NBi));
+      const int i = lane & 15, q = lane >> 4;
+      o[e] = kernel_at(P, flat, kl, 16 * nb + i, 16 * grp + 4 * q + r);
+    }
+    return;
+  }
   if (seg >= 3 * nm + 2) {
     if (P.prec != 0) return;
     if (seg < 4 * nm + 2) {
@@ -206,7 +223,7 @@ extern "C" int32_t nerf_mlp_pack(const nerf_mlp_desc* desc, int32_t precision, c
                                  void* packed, void* stream) {
   NERF_PLAN_OR_FAIL(desc, precision, P, "nerf_mlp_pack");
   NERF_CHECK_ARG(flat && packed, "nerf_mlp_pack: null pointer");
-  dim3 grid(64, 4 * P.n_mfma + 3), block(256);
+  dim3 grid(64, 5 * P.n_mfma + 3), block(256);
   pack_kernel<<<grid, block, 0, as_stream(stream)>>>(P, flat, reinterpret_cast<char*>(packed));
   NERF_LAUNCH_CHECK("nerf_mlp_pack");
   return NERF_OK;

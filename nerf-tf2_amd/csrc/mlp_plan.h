@@ -106,6 +106,10 @@ struct MlpPlan {
   // k-steps of every output block of 16), and both heads in natural feature
   // order (sigma: W weights + bias; rgb: [W/2][4] + bias[4]); -1 for bf16
   int64_t p_fwd4[NERF_MAX_LAYERS], p_hs_nat, p_hr_nat;
+  // fp32 only: transposed A operands of the register-resident dX chain, layers
+  // m >= 1 (k-step groups of 16 output features of layer m, each over the W/16
+  // blocks of its hidden input; -1 otherwise)
+  int64_t p_bwd4[NERF_MAX_LAYERS];
   int S4X, S4D;  // encoding slots per lane quarter in that forward
   int prec;
   // stash (elements per sample; fp32 or bf16 elements) and dz (floats per sample)
@@ -243,6 +247,13 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
     const int ke4 = P.ks_enc[i] == 0 ? 0 : (P.enc_L[i] == LX ? P.S4X : P.S4D);
     const int kh4 = P.ks_hid[i] == 0 ? 0 : W / 4;
     b += (int64_t)(ke4 + kh4) * (2 * P.nb_out[i]) * 64 * 4;
+  }
+  for (int i = 0; i < P.n_mfma; ++i) {
+    P.p_bwd4[i] = -1;
+    if (prec != 0 || i == 0) continue;
+    align(256);
+    P.p_bwd4[i] = b;
+    b += (int64_t)(2 * P.nb_out[i]) * (W / 16) * 64 * 4 * 4;
   }
   if (prec == 0) {
     align(256);
