@@ -24,7 +24,6 @@
 namespace wsb {
 
 using namespace ws;
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void st4(bf16x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, (int)soff, 0);
@@ -79,8 +78,9 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   // per-lane byte offsets in the rows of sample block sb: the lane's dz row chunk
   // of each 16-feature group (+ 8 g features), its stash mask word, the row start
   auto dz_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * zps + 8 * g) * 2); };
-  // this lane's mask words: the D words of (wave, lane half) are consecutive in the row
-  auto mask_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps) * 2 + 4 * g * D); };
+  // this lane's ReLU bit words: the D 16-bit words of (block, lane half) are
+  // consecutive in the row (element st_mask + (2 block + g) D + layer)
+  auto mask_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps) * 2 + 2 * g * D); };
   auto row0 = [&](int sb) { return (uint32_t)((sb * 32 + j) * sps * 2); };
 
   bf16x8 wlo[kHalf][2], whi[kHalf][2];
@@ -178,10 +178,10 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   }
 
   // ---- bottleneck and trunk, unrolled: step s = 0 is W_D (input dz_bott), then W_{D-1} .. W_1.
-  // The forward's ReLU bit words of trunk layers 4q .. 4q+3 are 16 contiguous bytes
-  // per lane (the mask region's [wave][lane half][layer] layout): loaded once per
-  // sample block every four steps.
-  u32x4 mq[kBlocks];
+  // The forward's ReLU bit words of trunk layers 4q .. 4q+3 of one block are 8
+  // contiguous bytes per lane (the mask region's [block][lane half][layer] layout):
+  // loaded for both of this wave's blocks once per sample block every four steps.
+  u32x2 mq[kBlocks][2];
   static_for<0, D>([&](auto Sc) {
     constexpr int s = decltype(Sc)::value;
     constexpr int layer = D - s;          // packed layer whose transpose is applied
@@ -196,14 +196,24 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
     if constexpr ((ml & 3) == 3) {
 #pragma unroll
       for (int sb = 0; sb < kBlocks; ++sb)
-        mq[sb] = __builtin_amdgcn_raw_buffer_load_b128(rst, mask_row(sb),
-                                                       (int)((P.st_mask + 4 * wv * D + 2 * (ml & ~3)) * 2), 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          mq[sb][i] = __builtin_amdgcn_raw_buffer_load_b64(
+              rst, mask_row(sb), (int)((P.st_mask + 2 * (nb0 + i) * D + (ml & ~3)) * 2), 0);
     }
+    // the 32-bit word of both blocks, as the epilogue reads it: block i's 16 bits
+    // at 8 i + {0..7} and 16 + 8 i + {0..7}
     uint32_t mbits[kBlocks];
 #pragma unroll
     for (int sb = 0; sb < kBlocks; ++sb) {
-      const u32x4 q = mq[sb];   // copied out first: an element subscript of a captured vector array
-      mbits[sb] = q[ml & 3];
+      uint32_t w = 0;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const u32x2 q = mq[sb][i];  // copied out first: an element subscript of a captured vector array
+        const uint32_t h = (q[(ml >> 1) & 1] >> (16 * (ml & 1))) & 0xFFFFu;
+        w |= ((h & 0xFFu) | ((h & 0xFF00u) << 8)) << (8 * i);
+      }
+      mbits[sb] = w;
     }
     if constexpr (s == 0) {
       // accumulators start from the sigma-head term W_sigma dz_sigma

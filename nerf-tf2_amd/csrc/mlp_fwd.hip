@@ -1,9 +1,8 @@
 // Fused positional-encoding + MLP forward (model.py:289-394) on MFMA.
 //
 // The reference topology (8 x 256 with the rgb branch) runs the register-
-// resident kernel of mlp_fwd_f32r.h in fp32 and the weight-stationary kernels
-// in bf16 (eight waves per workgroup for inference, mlp_fwd_bf16ws8.h; four,
-// with the training stash, mlp_fwd_bf16ws.h).  The fp32 kernel below serves the other topologies
+// resident kernel of mlp_fwd_f32r.h in fp32 and the eight-wave weight-
+// stationary kernel of mlp_fwd_bf16ws8.h in bf16.  The fp32 kernel below serves the other topologies
 // (the proposal and finite-difference nets).  One wave owns 32 samples and
 // carries them through every layer:
 //   * the encoding is computed in registers directly in B-operand slots;
@@ -179,13 +178,9 @@ int nerf_dispatch_fwd(const MlpPlan& P, const void* packed, int64_t M, const flo
       return launch_fwd_f32<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr,   \
                                                   rgb, sigma, stash, s);                      \
     if constexpr (W_ == 256 && LX_ == 10 && LD_ == 4) {                                       \
-      if (!stash) {                                                                           \
-        launch_fwd_bf16ws8<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr, rgb, \
-                                                 sigma, s);                                   \
-        return 0;                                                                             \
-      }                                                                                       \
-      return launch_fwd_bf16ws<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr, \
-                                                     rgb, sigma, stash, s);                   \
+      launch_fwd_bf16ws8<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr, rgb,   \
+                                               sigma, stash, s);                              \
+      return 0;                                                                               \
     }                                                                                         \
     return launch_fwd_bf16<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr,    \
                                                  rgb, sigma, stash, s);                       \

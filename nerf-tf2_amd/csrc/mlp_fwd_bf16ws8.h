@@ -1,5 +1,5 @@
-// Weight-stationary BF16 render forward with eight waves per workgroup (W = 256,
-// no training stash).
+// Weight-stationary BF16 forward with eight waves per workgroup (W = 256), with
+// or without the training stash.
 //
 // The layout is that of mlp_fwd_bf16ws.h (128 samples per workgroup, the
 // activations of a layer in a double-buffered LDS tile in the MFMA B layout,
@@ -14,8 +14,15 @@
 // flight); the second half runs them as two pairs, so the first pair's
 // epilogue overlaps the second pair's MFMAs.  The sigma head, the encodings,
 // the rgb branch (four 32-feature blocks) and the colours are done by waves
-// 0-3, as in the four-wave kernel.  Measured against the four-wave kernel on
-// the same box: 7.20 -> 6.54 ms per 32,768-ray render launch.  (At this
+// 0-3.  Measured against a four-wave kernel (two blocks per wave, one wave per
+// SIMD) on the same box: 7.20 -> 6.54 ms per 32,768-ray render launch.
+//
+// Training stash: the encodings and every layer's output rows (bf16, natural
+// feature order, 32-byte row pieces through a buffer resource over the
+// workgroup's rows, so rows past M are dropped), and the ReLU bits of the trunk
+// layers: per 32-feature block nb, lane half g and layer l one 16-bit word
+// (element st_mask + (2 nb + g) D + l of the row); a wave keeps its block's
+// words for four layers and writes them as one 8-byte store per sample block.  (At this
 // register budget the compiler issues each B operand read two or three MFMAs
 // ahead rather than five; pinning the ring with scheduling-group barriers
 // changed nothing measurable, the other wave on the SIMD covers the wait.)
@@ -67,15 +74,17 @@ __device__ __forceinline__ void run_steps(floatx16 (&acc)[kBlocks], const bf16x8
 
 }  // namespace ws8
 
-template <int W, int D, int SKIP, int LX, int LD>
+template <int W, int D, int SKIP, int LX, int LD, bool STASH>
 __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
     MlpPlan P, const char* __restrict__ pk, int64_t M, const float* __restrict__ xyz,
     const float* __restrict__ dirs, const float* __restrict__ ro, const float* __restrict__ rd,
-    const float* __restrict__ tv, int spr, float* __restrict__ rgb_out, float* __restrict__ sig_out) {
+    const float* __restrict__ tv, int spr, float* __restrict__ rgb_out, float* __restrict__ sig_out,
+    __bf16* __restrict__ stash) {
   using namespace ws8;
   using C = NetC<W, D, SKIP, LX, LD>;
   static_assert(W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D,
                 "weight-stationary kernel is for the 8x256 NeRF MLP");
+  static_assert(D % 4 == 0, "ReLU bit words are written four layers at a time");
   __shared__ bf16x8 s_act[2][kBlocks * kSlab];  // 2 x 64 KB
   __shared__ bf16x8 s_ex[kBlocks * kTX * 64];   // 16 KB
   __shared__ bf16x8 s_ed[kBlocks * kTD * 64];   // 8 KB
@@ -86,6 +95,16 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
   const int nb0 = wv;  // this wave's output block of a 256-wide layer
   const uint32_t lane_off = 16u * (uint32_t)lane, g_off = 16u * (uint32_t)g;
   const __amdgpu_buffer_rsrc_t rs = ws::packed_rsrc(pk, P.packed_bytes);
+  // the stash rows of this workgroup as a buffer resource (rows past M fall outside)
+  const int64_t wg_rows = M - base < kBlocks * 32 ? M - base : kBlocks * 32;
+  const __amdgpu_buffer_rsrc_t rst =
+      STASH ? __builtin_amdgcn_make_buffer_rsrc(stash + base * P.stash_per_sample, 0,
+                                                (int)(wg_rows * P.stash_per_sample * 2), 0x00020000)
+            : rs;
+  auto row_off = [&](int sb) { return (uint32_t)((sb * 32 + j) * P.stash_per_sample * 2); };
+  auto st16 = [&](bf16x8 v, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(ws::u32x4, v), rst, off, 0, 0);
+  };
   auto load_half = [&](bf16x8 (&w)[kHalf], int m, bool enc, bool second) {
     if (enc) {
       constexpr int H = (kT + kTX) / 2;
@@ -127,10 +146,29 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
     for (int t = 0; t < kTX; ++t) s_ex[(wv * kTX + t) * 64 + lane] = exb[t];
 #pragma unroll
     for (int t = 0; t < kTD; ++t) s_ed[(wv * kTD + t) * 64 + lane] = edb[t];
+    if (STASH) {
+      __bf16* r = stash + mm * P.stash_per_sample;
+      store_slots_bf16<C::KX>(ex, r + P.st_encx + g * C::KX);
+      store_slots_bf16<C::KD>(ed, r + P.st_encd + g * C::KD);
+    }
   }
 
-  // Epilogue of sample block sb: bf16, ReLU, LDS (B layout: k-steps 2 nb0, 2 nb0 + 1).
-  auto epilogue = [&](lds_b8* out, bool relu, int sb) {
+  // ReLU bit words of this wave's block for four trunk layers per sample block:
+  // layer 4 q + t in half t % 2 of mw[sb][t / 2]
+  uint32_t mw[kBlocks][2];
+  auto store_masks = [&](int q) {
+#pragma unroll
+    for (int sb = 0; sb < kBlocks; ++sb) {
+      const ws::u32x2 v = {mw[sb][0], mw[sb][1]};
+      __builtin_amdgcn_raw_buffer_store_b64(
+          v, rst, row_off(sb) + (uint32_t)((P.st_mask + (2 * nb0 + g) * D + 4 * q) * 2), 0, 0);
+    }
+  };
+
+  // Epilogue of sample block sb: bf16, ReLU, LDS (B layout: k-steps 2 nb0, 2 nb0 + 1),
+  // and with the stash the row pieces at st_off and (trunk layers, MASK) the ReLU bits.
+  auto epilogue = [&](lds_b8* out, bool relu, int64_t st_off, auto mask_c, int layer, int sb) {
+    constexpr bool MASK = decltype(mask_c)::value;
     bf16x8 lo = acc_to_bf16(acc[sb], 0), hi = acc_to_bf16(acc[sb], 1);
     if (relu) {
       lo = ws::relu_bf16(lo);
@@ -138,6 +176,18 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
     }
     out[sb * kSlab] = lo;
     out[sb * kSlab + 64] = hi;
+    if constexpr (STASH) {
+      // as row chunks: features 32 nb0 + 16 h + 8 g .. + 7 (lo: h = 0, hi: h = 1)
+      const uint32_t o = row_off(sb) + (uint32_t)((st_off + 32 * nb0 + 8 * g) * 2);
+      st16(ws::swap_halves(lo), o);
+      st16(ws::swap_halves(hi), o + 32);
+      if constexpr (MASK) {
+        const uint32_t b = ws::nz_bits(lo, hi);  // bits 0..7 and 16..23
+        const uint32_t v = (b & 0xFFu) | ((b >> 8) & 0xFF00u);
+        uint32_t& w = mw[sb][(layer >> 1) & 1];
+        w = (layer & 1) ? ((w & 0xFFFFu) | (v << 16)) : v;
+      }
+    }
   };
 
   __syncthreads();  // encodings visible
@@ -145,7 +195,8 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
   lds_b8* const ex_l = ws::lds_lane(s_ex, 0, lane);
   {
     lds_b8* const out_l = ws::lds_lane(s_act[0], 2 * nb0 * 64, lane);
-    run_steps<kT, kTX, true, true>(acc, whi, cb, nullptr, ex_l, [&](int sb) { epilogue(out_l, true, sb); });
+    run_steps<kT, kTX, true, true>(acc, whi, cb, nullptr, ex_l,
+                                   [&](int sb) { epilogue(out_l, true, P.st_h[0], std::true_type{}, 0, sb); });
   }
   load_half(whi, 1, SKIP == 0, true);  // layer 1's second half and bias
   cb = ws::bias_acc(rs, P.p_bias[1], nb0, g_off);
@@ -191,7 +242,11 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
     } else {
       if (wv < 4) load_w1<kHalf, 0>(wlo, rs, P.p_fwd[D + 1], C::NBR, wv, lane_off);
     }
-    run_steps<H, H, false, true>(acc, whi, cb, in_l, ex_l, [&](int sb) { epilogue(out_l, l < D, sb); });
+    const int64_t st_off = l < D ? P.st_h[l] : P.st_bott;
+    run_steps<H, H, false, true>(acc, whi, cb, in_l, ex_l, [&](int sb) {
+      epilogue(out_l, l < D, st_off, std::integral_constant<bool, (l < D)>{}, l, sb);
+    });
+    if constexpr (STASH && l < D && (l & 3) == 3) store_masks(l >> 2);
     if constexpr (l < D) {
       load_half(whi, l + 1, l + 1 == SKIP + 1, true);  // whi is free
     } else {
@@ -217,9 +272,19 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
       for (int t = 0; t < kT; ++t) a = mfma_bf16(wk(t), in_l[sb * kSlab + t * 64], a);
 #pragma unroll
       for (int t = 0; t < kTD; ++t) a = mfma_bf16(wk(kT + t), ed_l[(sb * kTD + t) * 64], a);
-      // ReLU, rounded to bf16, then the fp32 head partials
+      // ReLU, rounded to bf16 (the stashed activation), then the fp32 head partials
 #pragma unroll
       for (int r = 0; r < 16; ++r) a[r] = (float)(__bf16)(a[r] > 0.f ? a[r] : 0.f);
+      if (STASH) {
+        const uint32_t o = row_off(sb) + (uint32_t)((P.st_rbr + 32 * wv + 8 * g) * 2);
+        const bf16x8 lo = acc_to_bf16(a, 0), hi = acc_to_bf16(a, 1);
+        st16(ws::swap_halves(lo), o);
+        st16(ws::swap_halves(hi), o + 32);
+        // ReLU bits of rgb-branch block wv: bits 0..7 and 16..23 of nz_bits as 16 bits
+        const uint32_t b = ws::nz_bits(lo, hi);
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)((b & 0xFFu) | ((b >> 8) & 0xFF00u)), rst,
+                                              row_off(sb) + (uint32_t)((P.st_mask + 16 * D + 2 * wv + g) * 2), 0, 0);
+      }
       float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
@@ -257,8 +322,13 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
 template <int W, int D, int SKIP, int LX, int LD>
 void launch_fwd_bf16ws8(const MlpPlan& P, const void* packed, int64_t M, const float* xyz, const float* dirs,
                         const float* ro, const float* rd, const float* tv, int spr, float* rgb, float* sigma,
-                        hipStream_t s) {
+                        void* stash, hipStream_t s) {
   const int64_t nwg = (M + 127) / 128;
-  mlp_fwd_bf16ws8_kernel<W, D, SKIP, LX, LD><<<(unsigned)nwg, 512, 0, s>>>(
-      P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma);
+  if (stash)
+    mlp_fwd_bf16ws8_kernel<W, D, SKIP, LX, LD, true><<<(unsigned)nwg, 512, 0, s>>>(
+        P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma,
+        reinterpret_cast<__bf16*>(stash));
+  else
+    mlp_fwd_bf16ws8_kernel<W, D, SKIP, LX, LD, false><<<(unsigned)nwg, 512, 0, s>>>(
+        P, reinterpret_cast<const char*>(packed), M, xyz, dirs, ro, rd, tv, spr, rgb, sigma, nullptr);
 }

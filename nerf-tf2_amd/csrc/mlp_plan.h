@@ -115,9 +115,9 @@ struct MlpPlan {
   // stash (elements per sample; fp32 or bf16 elements) and dz (floats per sample)
   int64_t st_encx, st_h[NERF_MAX_LAYERS], st_bott, st_rbr, st_encd, stash_per_sample;
   // ReLU bit masks of the weight-stationary bf16 kernels (W = 256 with an rgb
-  // branch; -1 otherwise): per wave w and lane half g, D uint32 words, one per
-  // trunk layer (words 2 ((2 w + g) D + l)), 16 D elements in all; then 8
-  // elements for the rgb branch (one uint16 per 32-feature block and lane half)
+  // branch; -1 otherwise): per 32-feature block nb and lane half g, D uint16
+  // words, one per trunk layer (element (2 nb + g) D + l), 16 D elements in all;
+  // then 8 elements for the rgb branch (one uint16 per block and lane half)
   int64_t st_mask;
   int64_t dz_h[NERF_MAX_LAYERS], dz_bott, dz_rbr, dz_sigma, dz_rgb, dz_per_sample;
 };
