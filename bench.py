@@ -43,8 +43,8 @@ RENDER_FLOP_PER_RAY = (64 + 192) * MLP_FLOP_PER_SAMPLE
 # HBM bytes per fine-model MLP launch (FETCH_SIZE + WRITE_SIZE, each scaled by the
 # same counter on a 1 GiB device copy), from the rocprofv3 --pmc runs whose commands
 # and figures are in profiles/mlp_fwd_f32r_render800.md and
-# profiles/mlp_fwd_bf16ws_render800.md.
-TRAFFIC_BYTES_PER_LAUNCH = {"fp32": 153.8e6, "bf16": 142.9e6}
+# profiles/mlp_fwd_bf16ws8_render800.md.
+TRAFFIC_BYTES_PER_LAUNCH = {"fp32": 153.8e6, "bf16": 142.5e6}
 
 
 def parse():
@@ -553,7 +553,7 @@ def _render_chunk(net, nerf, ro, rd, near, far, timed_kernel, events):
         rgb_c, sig_c = net.coarse_model.forward_rays(ro, rd, dc["t_vals"])
         pc = ru.post_process_model_output(rgb_c, sig_c, dc["t_vals"], net.white_bg)
         df = ru.create_input_batch_fine_model(net.params, ro, rd, pc["weights"], dc["bin_data"], dc["t_vals"],
-                                              materialize=False)
+                                              materialize=False, compensated=net.compensated_cdf)
         if timed_kernel:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
