@@ -84,9 +84,10 @@ def test_composite_lengths(nerf, oracle, S):
         _close(pp[k], ref[k], rtol=1e-4, floor=1e-6, what=f"{k} S={S}")
 
 
-@pytest.mark.parametrize("M", [1, 31, 33, 127, 129])
+@pytest.mark.parametrize("M", [1, 15, 17, 31, 33, 65, 127, 129])
 def test_mlp_tiny_batches(nerf, oracle, M):
-    """One partial sample block (32 samples per wave, 128 per BF16 workgroup)."""
+    """Partial waves and workgroups: the FP32 forward carries 16 samples per wave
+    and 64 per workgroup, the BF16 forward 128 per workgroup."""
     g = torch.Generator().manual_seed(M)
     xyz = (torch.rand((M, 3), generator=g) * 6 - 3).float()
     dirs = torch.nn.functional.normalize(torch.randn((M, 3), generator=g), dim=1).float()
