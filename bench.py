@@ -98,8 +98,8 @@ def main():
               file=sys.stderr, flush=True)
         args.no_graphs = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU; "
-              "the path shards by ray ranges but is not run on several GPUs.",
+        print("bench.py measures one GPU only (--gpus 1): the path shards by ray ranges, "
+              "but it is not run or measured on several GPUs.",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
