@@ -280,8 +280,12 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[x][y][i] = 0.f;
-  float bsum = 0.f;
-  const bool do_bias = a.bpart && tk == 0 && tid < kTile;
+  // bias sums from the loaded dz values: thread t adds its 16 columns of every
+  // row it loads; the 32 threads of a column group meet at the end
+  float bs[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) bs[e] = 0.f;
+  const bool do_bias = a.bpart && tk == 0;
   // loader mapping: 256 threads x 16 floats = 32 rows x 128 cols
   const int lr = tid >> 3, lc = (tid & 7) * 16;
   for (int64_t c0 = mb; c0 < me; c0 += kChunk) {
@@ -312,6 +316,7 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
       for (int e = 0; e < 4; ++e) {
         Xs[lr][col + e] = xv[e];
         Zs[lr][col + e] = zv[e];
+        bs[4 * v + e] += zv[e];
       }
     }
     __syncthreads();
@@ -328,10 +333,6 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = mfma_f32(av[x], bv[y], acc[x][y]);
     }
-    if (do_bias) {
-#pragma unroll 8
-      for (int rr = 0; rr < kChunk; ++rr) bsum += Zs[rr][tid];
-    }
     __syncthreads();
   }
   float* out = a.part + (int64_t)split * a.K * a.N;
@@ -345,7 +346,19 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
         const int nn = n0 + wn + 32 * y + r;
         if (kk < a.K && nn < a.N) out[(int64_t)kk * a.N + nn] = acc[x][y][i];
       }
-  if (do_bias && n0 + tid < a.N) a.bpart[(int64_t)split * a.N + n0 + tid] = bsum;
+  if (do_bias) {
+    // threads lr = 0..31 of column group t & 7 hold partial sums: through LDS, fixed order
+    float* red = &Xs[0][0];  // [32 rows][128 columns] fits the 32 x 132 tile
+#pragma unroll
+    for (int e = 0; e < 16; ++e) red[lr * 128 + lc + e] = bs[e];
+    __syncthreads();
+    if (tid < kTile) {
+      float v = 0.f;
+#pragma unroll 8
+      for (int rr = 0; rr < kChunk; ++rr) v += red[rr * 128 + tid];
+      if (n0 + tid < a.N) a.bpart[(int64_t)split * a.N + n0 + tid] = v;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
