@@ -6,10 +6,12 @@
 // blocks of 16 input features of layer l, the k-steps run over its outputs, and
 // dz_l is the B operand in the accumulator layout of the step before (lane (j,
 // q) holds features 16 nb + 4 q + r of sample j), with the transposed weights
-// packed in that k order.  A wave needs about 200 registers, so two waves
-// share each SIMD.  The ReLU masks come from the stashed activation rows and
-// every dz_l is written row-major for the weight GEMMs, both as 16-byte buffer
-// accesses over the wave's 16 rows (rows past M fall outside the resources).
+// packed in that k order.  A wave needs under 200 registers, so two waves
+// share each SIMD.  The ReLU masks are the forward's branch bits (two words per
+// lane and trunk layer, one for the rgb branch, all loaded up front), so no
+// activation row is read; every dz_l is written row-major for the weight GEMMs
+// as 16-byte buffer stores over the wave's 16 rows (rows past M fall outside
+// the resources).
 //   rgb = sigmoid(z):  dz_rgb = d_rgb * rgb (1 - rgb)
 //   sigma = relu(z):   dz_sigma = d_sigma * [sigma > 0]
 //   dz = dh * [h > 0] for the ReLU layers; the bottleneck is linear.
@@ -22,17 +24,15 @@ __device__ __forceinline__ void store_f4(const floatx4& v, __amdgpu_buffer_rsrc_
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_cvt, v), r, off, 0, 0);
 }
 
-// dz[nb] = dh[nb] * [h > 0] over blocks 0 .. NBO-1, h the stashed row at st_off;
-// then the dz row at dz_off.
+// dz[nb] = dh[nb] * [h > 0] over blocks 0 .. NBO-1, with the forward's branch bits
+// (bit 4 (nb % 8) + r of w[nb / 8]); then the dz row at dz_off.
 template <int NBO>
-__device__ __forceinline__ void mask_store(floatx4 (&dz)[16], __amdgpu_buffer_rsrc_t rst, uint32_t row_s,
-                                           int64_t st_off, __amdgpu_buffer_rsrc_t rdz, uint32_t row_z,
-                                           int64_t dz_off) {
+__device__ __forceinline__ void mask_store(floatx4 (&dz)[16], const uint32_t (&w)[2], __amdgpu_buffer_rsrc_t rdz,
+                                           uint32_t row_z, int64_t dz_off) {
 #pragma unroll
   for (int nb = 0; nb < NBO; ++nb) {
-    const floatx4 h = ws::ldf4(rst, row_s, (st_off + 16 * nb) * 4);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) dz[nb][r] = h[r] > 0.f ? dz[nb][r] : 0.f;
+    for (int r = 0; r < 4; ++r) dz[nb][r] = ((w[nb >> 3] >> (4 * (nb & 7) + r)) & 1u) ? dz[nb][r] : 0.f;
     store_f4(dz[nb], rdz, row_z + (uint32_t)((dz_off + 16 * nb) * 4));
   }
 }
@@ -62,8 +62,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const_cast<float*>(stash) + m0 * P.stash_per_sample, 0, (int)(w_rows * P.stash_per_sample * 4), 0x00020000);
   const __amdgpu_buffer_rsrc_t rdz =
       __builtin_amdgcn_make_buffer_rsrc(dz + m0 * P.dz_per_sample, 0, (int)(w_rows * P.dz_per_sample * 4), 0x00020000);
-  const uint32_t row_s = (uint32_t)(j * P.stash_per_sample * 4) + q_off;
+  const uint32_t row_s = (uint32_t)(j * P.stash_per_sample * 4);
   const uint32_t row_z0 = (uint32_t)(j * P.dz_per_sample * 4), row_z = row_z0 + q_off;
+  // the forward's ReLU bits of every trunk layer and of the rgb branch
+  uint32_t mw[D][2], mr[2] = {0u, 0u};
+#pragma unroll
+  for (int l = 0; l < D; ++l) {
+    const ws::u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rst, row_s + (uint32_t)((P.st_mask + 8 * l + 2 * q) * 4), 0, 0);
+    mw[l][0] = v[0];
+    mw[l][1] = v[1];
+  }
+  mr[0] = __builtin_amdgcn_raw_buffer_load_b32(rst, row_s + (uint32_t)((P.st_mask + 8 * D + q) * 4), 0, 0);
 
   const float sg = sigma[mm];
   const float ds = valid ? d_sigma[mm] : 0.f;
@@ -93,7 +102,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const floatx4 w = ws::ldf4(rs, 4u * q_off, P.p_hr_nat + 16 * (16 * nb + r));
       act[nb][r] = dzr[0] * w[0] + dzr[1] * w[1] + dzr[2] * w[2];
     }
-  mask_store<NR>(act, rst, row_s, P.st_rbr, rdz, row_z, P.dz_rbr);
+  mask_store<NR>(act, mr, rdz, row_z, P.dz_rbr);
   // dh_bott = W_{D+1}[:W] dz_rbr; the bottleneck is linear, so dz_bott = dh_bott
   layer<16, 0, NR>(acc, act, nullptr, rs, lane_off, P.p_bwd4[D + 1]);
 #pragma unroll
@@ -109,14 +118,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
     for (int r = 0; r < 4; ++r) act[nb][r] = acc[nb][r] + w[r] * dzs;
   }
-  mask_store<16>(act, rst, row_s, P.st_h[D - 1], rdz, row_z, P.dz_h[D - 1]);
+  mask_store<16>(act, mw[D - 1], rdz, row_z, P.dz_h[D - 1]);
   // trunk, top-down: dh_{l-1} = W_l[:W] dz_l, dz_{l-1} = dh_{l-1} [h_{l-1} > 0]
   ws::static_for<1, D>([&](auto Lc) {
     constexpr int l = D - decltype(Lc)::value;  // D-1 .. 1
     layer<16, 0, NH>(acc, act, nullptr, rs, lane_off, P.p_bwd4[l]);
 #pragma unroll
     for (int nb = 0; nb < 16; ++nb) act[nb] = acc[nb];
-    mask_store<16>(act, rst, row_s, P.st_h[l - 1], rdz, row_z, P.dz_h[l - 1]);
+    mask_store<16>(act, mw[l - 1], rdz, row_z, P.dz_h[l - 1]);
   });
 }
 

@@ -17,7 +17,9 @@
 // per-sample reductions over the four lane quarters are VALU dot products and
 // two cross-lane shuffles.  The training stash gets the same rows as from the
 // 32-sample kernel (hidden rows in natural feature order, the encodings in the
-// lane-half slot layout), written by buffer stores over the wave's 16 rows.
+// lane-half slot layout), written by buffer stores over the wave's 16 rows, and
+// each ReLU layer's branch bits (64 per lane: act > 0), which the dX chain
+// reads instead of the activation rows.
 #pragma once
 #include "mlp_kernels.h"
 #include "mlp_fwd_bf16ws.h"  // ws::packed_rsrc, ws::ldf4, ws::static_for
@@ -124,6 +126,16 @@ __device__ __forceinline__ void store_rows(const floatx4 (&act)[16], __amdgpu_bu
                                            row_q + (uint32_t)((st_off + 16 * nb) * 4), 0, 0);
 }
 
+// ReLU branch bits of act[0 .. NBO): bit 4 (nb % 8) + r of word nb / 8 is act[nb][r] > 0.
+template <int NBO>
+__device__ __forceinline__ void relu_bits(const floatx4 (&act)[16], uint32_t (&w)[2]) {
+  w[0] = w[1] = 0u;
+#pragma unroll
+  for (int nb = 0; nb < NBO; ++nb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[nb >> 3] |= (act[nb][r] > 0.f ? 1u : 0u) << (4 * (nb & 7) + r);
+}
+
 // The encoding of this lane's quarter slots into the lane-half slot layout of the
 // stash (K slots per half, all 2 K written: pads as zeros by quarter 0).
 template <int L, int S, int K>
@@ -201,8 +213,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   floatx4 act[16], acc[16];
   // dense_0 on the xyz encoding
   layer<16, GX, 0>(acc, act, encx, rs, lane_off, P.p_fwd4[0]);
+  // ReLU bits of trunk layer l: two words at element st_mask + 8 l + 2 q of the row
+  auto store_bits = [&](int l) {
+    uint32_t w[2];
+    relu_bits<16>(act, w);
+    const ws::u32x2 v = {w[0], w[1]};
+    __builtin_amdgcn_raw_buffer_store_b64(v, rst, row + (uint32_t)((P.st_mask + 8 * l + 2 * q) * 4), 0, 0);
+  };
   epilogue<16, true>(act, acc, rs, q_off, P.p_bias[0]);
-  if constexpr (STASH) store_rows<16>(act, rst, row_q, P.st_h[0]);
+  if constexpr (STASH) {
+    store_rows<16>(act, rst, row_q, P.st_h[0]);
+    store_bits(0);
+  }
   // dense_1 .. dense_{D-1}; the encoding joins the input after dense_SKIP
   ws::static_for<1, D>([&](auto Lc) {
     constexpr int l = decltype(Lc)::value;
@@ -211,7 +233,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     else
       layer<16, 0, NH>(acc, act, encx, rs, lane_off, P.p_fwd4[l]);
     epilogue<16, true>(act, acc, rs, q_off, P.p_bias[l]);
-    if constexpr (STASH) store_rows<16>(act, rst, row_q, P.st_h[l]);
+    if constexpr (STASH) {
+      store_rows<16>(act, rst, row_q, P.st_h[l]);
+      store_bits(l);
+    }
   });
   // sigma head on h_{D-1} (model.py:374-375)
   float zs = 0.f;
@@ -232,7 +257,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // dense_{D+1} on concat(bottleneck, direction encoding) (model.py:380-384): 8 blocks of 16
   layer<8, GD, NH>(acc, act, encd, rs, lane_off, P.p_fwd4[D + 1]);
   epilogue<8, true>(act, acc, rs, q_off, P.p_bias[D + 1]);
-  if constexpr (STASH) store_rows<8>(act, rst, row_q, P.st_rbr);
+  if constexpr (STASH) {
+    store_rows<8>(act, rst, row_q, P.st_rbr);
+    uint32_t w[2];
+    relu_bits<8>(act, w);
+    __builtin_amdgcn_raw_buffer_store_b32(w[0], rst, row + (uint32_t)((P.st_mask + 8 * D + q) * 4), 0, 0);
+  }
   // rgb head, sigmoid (model.py:386-387): weights [feature][4]
   float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll

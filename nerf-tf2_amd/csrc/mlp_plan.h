@@ -114,10 +114,14 @@ struct MlpPlan {
   int prec;
   // stash (elements per sample; fp32 or bf16 elements) and dz (floats per sample)
   int64_t st_encx, st_h[NERF_MAX_LAYERS], st_bott, st_rbr, st_encd, stash_per_sample;
-  // ReLU bit masks of the weight-stationary bf16 kernels (W = 256 with an rgb
-  // branch; -1 otherwise): per 32-feature block nb and lane half g, D uint16
-  // words, one per trunk layer (element (2 nb + g) D + l), 16 D elements in all;
-  // then 8 elements for the rgb branch (one uint16 per block and lane half)
+  // ReLU bit masks (W = 256 with an rgb branch; -1 otherwise), written by the
+  // forward for the dX chain.  bf16: per 32-feature block nb and lane half g, D
+  // uint16 words, one per trunk layer (element (2 nb + g) D + l), 16 D elements
+  // in all; then 8 elements for the rgb branch (one uint16 per block and lane
+  // half).  fp32 (register-resident kernels, lane quarter q holding features
+  // 16 nb + 4 q + r): per trunk layer l and quarter q two uint32 words (element
+  // 8 l + 2 q + h, bit 4 (nb - 8 h) + r for nb = 8 h .. 8 h + 7), 8 D elements;
+  // then one word per quarter for the rgb branch's 8 blocks (element 8 D + q)
   int64_t st_mask;
   int64_t dz_h[NERF_MAX_LAYERS], dz_bott, dz_rbr, dz_sigma, dz_rgb, dz_per_sample;
 };
