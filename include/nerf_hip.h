@@ -172,6 +172,15 @@ int64_t nerf_mlp_stash_floats_per_sample(const nerf_mlp_desc* desc);
 /* Elements of the per-sample gradient buffer (dz) used by backward /
  * weight_grads: float32 for NERF_PREC_F32, bf16 for NERF_PREC_BF16. */
 int64_t nerf_mlp_dz_floats_per_sample(const nerf_mlp_desc* desc);
+/* Samples per block of the stash and dz buffers for `precision` (-1 for an
+ * unsupported description): both hold M * (elements per sample) elements, cut
+ * into blocks of this many consecutive samples, each block stored chunk-major
+ * (1: plain rows; 32 for the 8x256 reference MLP in bf16, so that the kernels
+ * write and read whole 512-byte stretches).  A batch whose weight gradients are
+ * summed over several nerf_mlp_weight_grads calls (accumulate) is split at
+ * multiples of it.  No counterpart in the reference (TF keeps activations in
+ * its own tensors). */
+int32_t nerf_mlp_sample_block(const nerf_mlp_desc* desc, int32_t precision);
 
 /* Pack flat float32 weights into the MFMA operand layout (forward and
  * transposed-for-backward copies, in fp32 or bf16). */
@@ -201,7 +210,9 @@ int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precision, const vo
 
 /* Weight gradients: grad_flat (param_count floats, flat layout) = or +=
  * (accumulate != 0) sum over the M samples of X^T dz for every layer, and the
- * bias sums.  workspace: a device buffer of workspace_bytes bytes, at least
+ * bias sums.  stash and dz hold M samples as the forward and backward wrote
+ * them; a sub-range of a larger batch starts and ends on multiples of
+ * nerf_mlp_sample_block() (or at the batch's end).  workspace: a device buffer of workspace_bytes bytes, at least
  * nerf_mlp_wgrad_workspace_bytes(desc, M) (a bound for both precisions); a
  * smaller buffer than the call's launches use is rejected with
  * NERF_ERR_INVALID_ARG before anything is launched.

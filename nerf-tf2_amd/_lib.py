@@ -51,6 +51,7 @@ _SIGS = {
     "nerf_mlp_packed_bytes": (_i64, [_descp, _i32]),
     "nerf_mlp_stash_floats_per_sample": (_i64, [_descp]),
     "nerf_mlp_dz_floats_per_sample": (_i64, [_descp]),
+    "nerf_mlp_sample_block": (_i32, [_descp, _i32]),
     "nerf_mlp_pack": (_i32, [_descp, _i32, _vp, _vp, _vp]),
     "nerf_mlp_forward": (_i32, [_descp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
                                 _vp, _vp]),

@@ -397,10 +397,23 @@ struct Wgrad16Args {
   int64_t ldx;
   const __bf16* Z;
   int64_t ldz;
-  int64_t M, m_per_split;
+  int64_t M, m_per_split;  // m_per_split: a multiple of the sample block
   int splits, njobs;
+  int sb_shift;            // log2 of the stash / dz sample block (MlpPlan::sblk)
   Wgrad16Job job[kMaxJobs];
 };
+
+// Byte offset of the 8-element chunk at element e (a multiple of 8) of row m of a
+// split (rows [0, rows) from its first sample; splits start on sample-block
+// boundaries) in a blocked buffer with ld elements per sample; rows past the split
+// get an offset outside its buffer resource (they read zeros).
+__device__ __forceinline__ uint32_t blk_chunk_off(uint32_t m, int64_t e, int64_t ld, int64_t rows, int sh) {
+  if (m >= rows) return 0x80000000u;
+  const uint32_t b = (m >> sh) << sh;
+  const int64_t left = rows - b;
+  const int64_t n = left < (1 << sh) ? left : (1 << sh);
+  return (uint32_t)((b * ld + 8 * ((e >> 3) * n + (m - b))) * 2);
+}
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const __bf16* base, int64_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(base), 0, (int)bytes, 0x00020000);
@@ -454,14 +467,16 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const uint32_t m = (uint32_t)(c * kWC + 2 * (psx + PPX * i) + h);
-        xr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(rx, (uint32_t)((m * a.ldx + J.xoff + 8 * fgx) * 2), 0, 0);
+        xr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
+            rx, blk_chunk_off(m, J.xoff + 8 * fgx, a.ldx, rows, a.sb_shift), 0, 0);
       }
 #pragma unroll
     for (int i = 0; i < RZ; ++i)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const uint32_t m = (uint32_t)(c * kWC + 2 * (psz + PPZ * i) + h);
-        zr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(rz, (uint32_t)((m * a.ldz + J.zoff + 8 * fgz) * 2), 0, 0);
+        zr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
+            rz, blk_chunk_off(m, J.zoff + 8 * fgz, a.ldz, rows, a.sb_shift), 0, 0);
       }
   };
   auto store_chunk = [&](int buf) {
@@ -592,20 +607,19 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
   for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
 
   u32x4_ xr[2][R][2], zr[2][R][2];
-  // per-thread byte offsets; the row of each load (chunk, round, pair half) is added
-  // to the VGPR offset, which the buffer range check covers (rows past the split
-  // read zeros), not to the unchecked scalar offset
-  const uint32_t vx = (uint32_t)(((2 * ps) * a.ldx + J.xoff + 8 * fg) * 2);
-  const uint32_t vz = (uint32_t)(((2 * ps) * a.ldz + J.zoff + 8 * fg) * 2);
+  // per-load byte offsets in the VGPR offset, which the buffer range check covers
+  // (rows past the split read zeros), not in the unchecked scalar offset
   auto load_chunk = [&](auto set, int c) {
     constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const uint32_t row = (uint32_t)(c * kWC + 2 * PP * i + h);
-        xr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(rx, vx + row * (uint32_t)(a.ldx * 2), 0, 0);
-        zr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(rz, vz + row * (uint32_t)(a.ldz * 2), 0, 0);
+        const uint32_t row = (uint32_t)(c * kWC + 2 * PP * i + 2 * ps + h);
+        xr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(
+            rx, blk_chunk_off(row, J.xoff + 8 * fg, a.ldx, rows, a.sb_shift), 0, 0);
+        zr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(
+            rz, blk_chunk_off(row, J.zoff + 8 * fg, a.ldz, rows, a.sb_shift), 0, 0);
       }
   };
   auto store_chunk = [&](auto set, int buf) {
@@ -899,10 +913,13 @@ template <int W, int D, int SKIP, int LX, int LD>
 int launch_bwd_bf16_any(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,
                         const float* d_sigma, const float* rgb, const float* sigma, const void* stash,
                         void* dz, hipStream_t s) {
+  // the stash / dz layout (MlpPlan::sblk) must be the one the kernel reads and writes
   if constexpr (W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D)
-    return launch_bwd_bf16ws<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
+    return P.sblk != 32 ? -1
+                        : launch_bwd_bf16ws<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
   else
-    return launch_bwd_bf16<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
+    return P.sblk != 1 ? -1
+                       : launch_bwd_bf16<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
 }
 
 int dispatch_bwd(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,
@@ -911,8 +928,8 @@ int dispatch_bwd(const MlpPlan& P, const void* packed, int64_t M, const float* d
 #define NERF_BWD_INST(W_, D_, S_, LX_, LD_)                                                   \
   if (P.W == W_ && P.D == D_ && P.skip == S_ && P.LX == LX_ && P.LD == LD_) {                \
     if (P.prec == NERF_PREC_F32)                                                              \
-      return launch_bwd_f32<W_, D_, S_, LX_, LD_>(P, packed, M, d_rgb, d_sigma, rgb, sigma,   \
-                                                  stash, dz, s);                              \
+      return P.sblk != 1 ? -1 : launch_bwd_f32<W_, D_, S_, LX_, LD_>(P, packed, M, d_rgb,     \
+                                                  d_sigma, rgb, sigma, stash, dz, s);         \
     return launch_bwd_bf16_any<W_, D_, S_, LX_, LD_>(P, packed, M, d_rgb, d_sigma, rgb, sigma,\
                                                      stash, dz, s);                           \
   }
@@ -1071,8 +1088,10 @@ static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const v
       b.Z = reinterpret_cast<const __bf16*>(dz);
       b.ldz = P.dz_per_sample;
       b.M = M;
-      b.m_per_split = (M + sg - 1) / sg;
+      // splits start on sample-block boundaries of the stash / dz layout
+      b.m_per_split = ((M + sg - 1) / sg + P.sblk - 1) / P.sblk * P.sblk;
       b.splits = (int)sg;
+      b.sb_shift = P.sblk == 32 ? 5 : 0;
       b.njobs = n;
       float* wp = part;
       for (int i = 0; i < n; ++i) {

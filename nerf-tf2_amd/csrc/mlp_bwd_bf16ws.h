@@ -16,8 +16,11 @@
 // (one dword per wave, lane and layer), applied to the packed bf16 result.  dz
 // rows are written as 16-byte chunks of 8 consecutive features, swapped out of
 // the accumulator layout across lane halves (swap_halves).
-// Stash reads and dz writes go through buffer resources that end at the last
-// sample, so lanes past it read zeros and their stores are dropped.
+// The stash and dz buffers are blocked by 32 samples (MlpPlan::sblk): the 16-byte
+// row chunks the lanes (j, g) of a wave write for one sample block are 1 KB of
+// contiguous memory.  Stash reads and dz writes go through buffer resources over
+// the workgroup's rows; lanes past the last sample get a VGPR offset outside
+// them, so they read zeros and their stores are dropped.
 #pragma once
 #include "mlp_fwd_bf16ws.h"
 
@@ -25,8 +28,12 @@ namespace wsb {
 
 using namespace ws;
 
+// Stores take their whole offset in the VGPR offset (scalar offset 0): with a
+// scalar offset register the compiler puts no wait state between a 16-byte
+// store and a VALU write of its data registers, and on gfx950 such stores have
+// written another value for some lanes.
 __device__ __forceinline__ void st4(bf16x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff + (uint32_t)soff, 0, 0);
 }
 
 // v with the ReLU mask applied: the forward's bit mask w holds, for the pairs of
@@ -45,7 +52,7 @@ __device__ __forceinline__ uint32_t ld_u32(__amdgpu_buffer_rsrc_t r, uint32_t vo
 
 // Stores a 16-feature group given in the accumulator layout as row chunks.
 __device__ __forceinline__ void st_group(bf16x8 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, swap_halves(v)), r, voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, swap_halves(v)), r, voff + (uint32_t)soff, 0, 0);
 }
 
 }  // namespace wsb
@@ -60,7 +67,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   static_assert(W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D,
                 "weight-stationary backward is for the 8x256 NeRF MLP");
   static_assert(C::NBR == 4, "rgb branch of 128 features");
-  static_assert(D % 4 == 0, "ReLU bit words are read four layers at a time");
+  static_assert(D == 8, "ReLU bit words: (2 nb + g) D is one chunk per lane half, four layers per load");
   __shared__ bf16x8 s_act[2][kBlocks * kSlab];  // 2 x 64 KB
   __shared__ float s_dzs[kBlocks * 32];          // dz_sigma of the workgroup's samples
   const int lane = threadIdx.x & 63, j = lane & 31, g = lane >> 5;
@@ -75,13 +82,22 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
       const_cast<__bf16*>(stash + base * sps), 0, (int)(nrows * sps * 2), 0x00020000);
   const __amdgpu_buffer_rsrc_t rdz =
       __builtin_amdgcn_make_buffer_rsrc(dz + base * zps, 0, (int)(nrows * zps * 2), 0x00020000);
-  // per-lane byte offsets in the rows of sample block sb: the lane's dz row chunk
-  // of each 16-feature group (+ 8 g features), its stash mask word, the row start
-  auto dz_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * zps + 8 * g) * 2); };
-  // this lane's ReLU bit words: the D 16-bit words of (block, lane half) are
-  // consecutive in the row (element st_mask + (2 block + g) D + layer)
-  auto mask_row = [&](int sb) { return (uint32_t)(((sb * 32 + j) * sps) * 2 + 2 * g * D); };
-  auto row0 = [&](int sb) { return (uint32_t)((sb * 32 + j) * sps * 2); };
+  // Offsets in the blocked rows: element u + 8 ch + el of this lane's sample in
+  // sample block sb, u wave-uniform (u % 8 + el + the access's elements <= 8).  The
+  // lane part is the VGPR offset (out of range past the last sample), the uniform
+  // part the scalar offset.
+  int nrow[kBlocks];  // rows of each sample block (wave-uniform)
+  uint32_t sbase[kBlocks], zbase[kBlocks];
+#pragma unroll
+  for (int sb = 0; sb < kBlocks; ++sb) {
+    const int64_t left = M - base - 32 * sb;
+    nrow[sb] = __builtin_amdgcn_readfirstlane((int)(left < 32 ? (left > 0 ? left : 0) : 32));
+    sbase[sb] = j < nrow[sb] ? (uint32_t)((64 * sb * sps + 16 * j)) : 0x80000000u;
+    zbase[sb] = j < nrow[sb] ? (uint32_t)((64 * sb * zps + 16 * j)) : 0x80000000u;
+  }
+  auto sv = [&](int sb, int ch, int el) { return sbase[sb] + (uint32_t)(16 * nrow[sb] * ch + 2 * el); };
+  auto zv = [&](int sb, int ch) { return zbase[sb] + (uint32_t)(16 * nrow[sb] * ch); };
+  auto uo = [&](int sb, int64_t u) { return (int64_t)(16 * nrow[sb] * (int)(u >> 3) + 2 * (int)(u & 7)); };
 
   bf16x8 wlo[kHalf][2], whi[kHalf][2];
   const floatx16 cb_unused[2] = {};
@@ -108,12 +124,11 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
       dzr[c] = dy * y * (1.f - y);
     }
     if (g == 0) s_dzs[wv * 32 + j] = dzs;
-    const uint32_t zr = dz_row(wv), r0 = row0(wv);
     if (g == 0) {
       const bf16x4 hz = {(__bf16)dzs, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f};
       const bf16x4 rz = {(__bf16)dzr[0], (__bf16)dzr[1], (__bf16)dzr[2], (__bf16)0.f};
-      st4(hz, rdz, zr, P.dz_sigma * 2);
-      st4(rz, rdz, zr, P.dz_rgb * 2);
+      st4(hz, rdz, zv(wv, 0), uo(wv, P.dz_sigma));
+      st4(rz, rdz, zv(wv, 0), uo(wv, P.dz_rgb));
     }
     const floatx4* hr4 = reinterpret_cast<const floatx4*>(pk + P.p_head_rgb) + g * 16 * C::NBR;
     lds_b8* const out = lds_lane(s_act[0], wv * kSlab, lane);
@@ -126,14 +141,14 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
         a[i] = dzr[0] * w[0] + dzr[1] * w[1] + dzr[2] * w[2];
       }
       // the forward's 16 ReLU bits of rgb-branch block nb (lane halves in one dword)
-      const uint32_t hw = ld_u32(rst, r0, (P.st_mask + 16 * D + 2 * nb) * 2);
+      const uint32_t hw = ld_u32(rst, sv(wv, 0, 0), uo(wv, P.st_mask + 16 * D + 2 * nb));
       const uint32_t u = g ? hw >> 16 : hw & 0xFFFFu;
       const uint32_t bits = (u & 0xFFu) | ((u & 0xFF00u) << 8);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const bf16x8 v = mask_bits(acc_to_bf16(a, h), bits, 4 * h);
         out[(2 * nb + h) * 64] = v;
-        st_group(v, rdz, zr, (P.dz_rbr + 32 * nb + 16 * h) * 2);
+        st_group(v, rdz, zv(wv, g), uo(wv, P.dz_rbr + 32 * nb + 16 * h));
       }
     }
   }
@@ -144,7 +159,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
   // at dz_off.
   auto epilogue = [&](auto masked_c, auto lds_c, lds_b8* out, uint32_t bits, int64_t dz_off, int sb) {
     constexpr bool MASKED = decltype(masked_c)::value, TO_LDS = decltype(lds_c)::value;
-    const uint32_t zr = dz_row(sb);
+    const uint32_t zr = zv(sb, g);  // lane half g: the second chunk of each 16-feature group
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int64_t f0 = 32 * (nb0 + i);
@@ -153,7 +168,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
         bf16x8 v = acc_to_bf16(acc[sb][i], h);
         if constexpr (MASKED) v = mask_bits(v, bits, 8 * i + 4 * h);
         if constexpr (TO_LDS) out[sb * kSlab + (2 * i + h) * 64] = v;
-        st_group(v, rdz, zr, (dz_off + f0 + 16 * h) * 2);
+        st_group(v, rdz, zr, uo(sb, dz_off + f0 + 16 * h));
       }
     }
   };
@@ -199,7 +214,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16ws_kernel(
 #pragma unroll
         for (int i = 0; i < 2; ++i)
           mq[sb][i] = __builtin_amdgcn_raw_buffer_load_b64(
-              rst, mask_row(sb), (int)((P.st_mask + 2 * (nb0 + i) * D + (ml & ~3)) * 2), 0);
+              rst, sv(sb, g, 0), (int)uo(sb, P.st_mask + 2 * (nb0 + i) * D + (ml & ~3)), 0);
     }
     // the 32-bit word of both blocks, as the epilogue reads it: block i's 16 bits
     // at 8 i + {0..7} and 16 + 8 i + {0..7}

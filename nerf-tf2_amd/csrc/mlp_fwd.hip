@@ -174,14 +174,17 @@ int nerf_dispatch_fwd(const MlpPlan& P, const void* packed, int64_t M, const flo
                       int spr, float* rgb, float* sigma, void* stash, hipStream_t s) {
 #define NERF_FWD_INST(W_, D_, S_, LX_, LD_)                                                   \
   if (P.W == W_ && P.D == D_ && P.skip == S_ && P.LX == LX_ && P.LD == LD_) {                \
+    /* the stash layout (MlpPlan::sblk) must be the one the kernel writes */                 \
     if (P.prec == NERF_PREC_F32)                                                              \
-      return launch_fwd_f32<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr,   \
-                                                  rgb, sigma, stash, s);                      \
+      return P.sblk != 1 ? -1 : launch_fwd_f32<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, \
+                                                  ro, rd, tv, spr, rgb, sigma, stash, s);     \
     if constexpr (W_ == 256 && LX_ == 10 && LD_ == 4) {                                       \
+      if (P.sblk != 32) return -1;                                                            \
       launch_fwd_bf16ws8<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr, rgb,   \
                                                sigma, stash, s);                              \
       return 0;                                                                               \
     }                                                                                         \
+    if (P.sblk != 1) return -1;                                                               \
     return launch_fwd_bf16<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr,    \
                                                  rgb, sigma, stash, s);                       \
   }

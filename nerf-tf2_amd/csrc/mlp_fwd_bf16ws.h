@@ -95,6 +95,12 @@ __device__ __forceinline__ bf16x8 swap_halves(bf16x8 v) {
 
 typedef unsigned short ushortx2 __attribute__((ext_vector_type(2)));
 
+// Four floats rounded to bf16 (as __builtin_convertvector) as the dwords of an 8-byte store.
+__device__ __forceinline__ u32x2 bf16x4_bits(float a, float b, float c, float d) {
+  const floatx4_ x = {a, b, c, d};
+  return __builtin_bit_cast(u32x2, __builtin_convertvector(x, bf16x4));
+}
+
 // Nonzero bits of a block's 16 accumulator elements (lo: elements 0..7, hi:
 // 8..15; non-negative bf16 after ReLU, so nonzero == positive): the pair
 // (2k, 2k+1) goes to bits k and 16 + k, k = 0..7.

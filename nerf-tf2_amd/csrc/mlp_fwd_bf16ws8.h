@@ -18,11 +18,14 @@
 // SIMD) on the same box: 7.20 -> 6.54 ms per 32,768-ray render launch.
 //
 // Training stash: the encodings and every layer's output rows (bf16, natural
-// feature order, 32-byte row pieces through a buffer resource over the
-// workgroup's rows, so rows past M are dropped), and the ReLU bits of the trunk
-// layers: per 32-feature block nb, lane half g and layer l one 16-bit word
-// (element st_mask + (2 nb + g) D + l of the row); a wave keeps its block's
-// words for four layers and writes them as one 8-byte store per sample block.  (At this
+// feature order), and the ReLU bits of the trunk layers: per 32-feature block
+// nb, lane half g and layer l one 16-bit word (element st_mask + (2 nb + g) D +
+// l of the row); a wave keeps its block's words for four layers and writes them
+// as one 8-byte store per sample block.  The stash is blocked by 32 samples
+// (MlpPlan::sblk), so the two 16-byte row chunks a wave's lanes (j, g) write for
+// one sample block are 1 KB of contiguous memory, and each workgroup writes a
+// 64 KB stretch per layer.  Stores go through a buffer resource over the
+// workgroup's rows; lanes past M get an offset outside it and are dropped.  (At this
 // register budget the compiler issues each B operand read two or three MFMAs
 // ahead rather than five; pinning the ring with scheduling-group barriers
 // changed nothing measurable, the other wave on the SIMD covers the wait.)
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
   using C = NetC<W, D, SKIP, LX, LD>;
   static_assert(W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D,
                 "weight-stationary kernel is for the 8x256 NeRF MLP");
-  static_assert(D % 4 == 0, "ReLU bit words are written four layers at a time");
+  static_assert(D == 8, "ReLU bit words: (2 nb + g) D is one chunk per lane half, four layers per store");
   __shared__ bf16x8 s_act[2][kBlocks * kSlab];  // 2 x 64 KB
   __shared__ bf16x8 s_ex[kBlocks * kTX * 64];   // 16 KB
   __shared__ bf16x8 s_ed[kBlocks * kTD * 64];   // 8 KB
@@ -101,9 +104,28 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
       STASH ? __builtin_amdgcn_make_buffer_rsrc(stash + base * P.stash_per_sample, 0,
                                                 (int)(wg_rows * P.stash_per_sample * 2), 0x00020000)
             : rs;
-  auto row_off = [&](int sb) { return (uint32_t)((sb * 32 + j) * P.stash_per_sample * 2); };
-  auto st16 = [&](bf16x8 v, uint32_t off) {
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(ws::u32x4, v), rst, off, 0, 0);
+  // Stash offsets (blocked by 32 samples): element u + 8 ch + el of this lane's
+  // sample in sample block sb of the workgroup, u wave-uniform with u % 8 + el +
+  // the store's elements <= 8: a lane part (out of range for rows past M, which
+  // drops the store) plus a uniform part, both in the VGPR offset; the scalar
+  // offset of these 8- and 16-byte stores stays 0.
+  const int sps = (int)P.stash_per_sample;
+  int nrow[kBlocks];  // rows of each sample block (wave-uniform)
+#pragma unroll
+  for (int sb = 0; sb < kBlocks; ++sb) {
+    const int64_t left = M - base - 32 * sb;
+    nrow[sb] = __builtin_amdgcn_readfirstlane((int)(left < 32 ? (left > 0 ? left : 0) : 32));
+  }
+  auto rows_in = [&](int sb) { return nrow[sb]; };
+  uint32_t lbase[kBlocks];  // this lane's row start in each sample block; out of range past M
+#pragma unroll
+  for (int sb = 0; sb < kBlocks; ++sb) lbase[sb] = j < nrow[sb] ? (uint32_t)(64 * sb * sps + 16 * j) : 0x80000000u;
+  auto voff = [&](int sb, int ch, int el) -> uint32_t {
+    return lbase[sb] + (uint32_t)(16 * rows_in(sb) * ch + 2 * el);
+  };
+  auto uoff = [&](int sb, int64_t u) { return (int)(16 * rows_in(sb) * (int)(u >> 3) + 2 * (int)(u & 7)); };
+  auto st16 = [&](bf16x8 v, uint32_t vo, int so) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(ws::u32x4, v), rst, vo + (uint32_t)so, 0, 0);
   };
   auto load_half = [&](bf16x8 (&w)[kHalf], int m, bool enc, bool second) {
     if (enc) {
@@ -147,9 +169,17 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
 #pragma unroll
     for (int t = 0; t < kTD; ++t) s_ed[(wv * kTD + t) * 64 + lane] = edb[t];
     if (STASH) {
-      __bf16* r = stash + mm * P.stash_per_sample;
-      store_slots_bf16<C::KX>(ex, r + P.st_encx + g * C::KX);
-      store_slots_bf16<C::KD>(ed, r + P.st_encd + g * C::KD);
+      // half g's slots start K / 8 chunks in (K a multiple of 8)
+      static_assert(C::KX % 8 == 0 && C::KD % 8 == 0, "encoding slots in whole chunks");
+      const uint32_t vx = voff(wv, g * (C::KX / 8), 0), vd = voff(wv, g * (C::KD / 8), 0);
+#pragma unroll
+      for (int e = 0; e < C::KX; e += 4)
+        __builtin_amdgcn_raw_buffer_store_b64(ws::bf16x4_bits(ex[e], ex[e + 1], ex[e + 2], ex[e + 3]), rst,
+                                              vx + (uint32_t)uoff(wv, P.st_encx + e), 0, 0);
+#pragma unroll
+      for (int e = 0; e < C::KD; e += 4)
+        __builtin_amdgcn_raw_buffer_store_b64(ws::bf16x4_bits(ed[e], ed[e + 1], ed[e + 2], ed[e + 3]), rst,
+                                              vd + (uint32_t)uoff(wv, P.st_encd + e), 0, 0);
     }
   }
 
@@ -160,8 +190,9 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
 #pragma unroll
     for (int sb = 0; sb < kBlocks; ++sb) {
       const ws::u32x2 v = {mw[sb][0], mw[sb][1]};
+      // (2 nb0 + g) D = 16 nb0 + 8 g: lane half g one chunk on
       __builtin_amdgcn_raw_buffer_store_b64(
-          v, rst, row_off(sb) + (uint32_t)((P.st_mask + (2 * nb0 + g) * D + 4 * q) * 2), 0, 0);
+          v, rst, voff(sb, g, 0) + (uint32_t)uoff(sb, P.st_mask + 2 * nb0 * D + 4 * q), 0, 0);
     }
   };
 
@@ -178,9 +209,9 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
     out[sb * kSlab + 64] = hi;
     if constexpr (STASH) {
       // as row chunks: features 32 nb0 + 16 h + 8 g .. + 7 (lo: h = 0, hi: h = 1)
-      const uint32_t o = row_off(sb) + (uint32_t)((st_off + 32 * nb0 + 8 * g) * 2);
-      st16(ws::swap_halves(lo), o);
-      st16(ws::swap_halves(hi), o + 32);
+      const uint32_t vo = voff(sb, g, 0);
+      st16(ws::swap_halves(lo), vo, uoff(sb, st_off + 32 * nb0));
+      st16(ws::swap_halves(hi), vo, uoff(sb, st_off + 32 * nb0 + 16));
       if constexpr (MASK) {
         const uint32_t b = ws::nz_bits(lo, hi);  // bits 0..7 and 16..23
         const uint32_t v = (b & 0xFFu) | ((b >> 8) & 0xFF00u);
@@ -276,14 +307,14 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) a[r] = (float)(__bf16)(a[r] > 0.f ? a[r] : 0.f);
       if (STASH) {
-        const uint32_t o = row_off(sb) + (uint32_t)((P.st_rbr + 32 * wv + 8 * g) * 2);
         const bf16x8 lo = acc_to_bf16(a, 0), hi = acc_to_bf16(a, 1);
-        st16(ws::swap_halves(lo), o);
-        st16(ws::swap_halves(hi), o + 32);
+        const uint32_t vo = voff(sb, g, 0);
+        st16(ws::swap_halves(lo), vo, uoff(sb, P.st_rbr + 32 * wv));
+        st16(ws::swap_halves(hi), vo, uoff(sb, P.st_rbr + 32 * wv + 16));
         // ReLU bits of rgb-branch block wv: bits 0..7 and 16..23 of nz_bits as 16 bits
         const uint32_t b = ws::nz_bits(lo, hi);
         __builtin_amdgcn_raw_buffer_store_b16((unsigned short)((b & 0xFFu) | ((b >> 8) & 0xFF00u)), rst,
-                                              row_off(sb) + (uint32_t)((P.st_mask + 16 * D + 2 * wv + g) * 2), 0, 0);
+                                              voff(sb, 0, g) + (uint32_t)uoff(sb, P.st_mask + 16 * D + 2 * wv), 0, 0);
       }
       float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll

@@ -218,6 +218,11 @@ extern "C" int64_t nerf_mlp_dz_floats_per_sample(const nerf_mlp_desc* desc) {
   if (nerf_plan_from_desc(desc, 0, &P)) return -1;
   return P.dz_per_sample;
 }
+extern "C" int32_t nerf_mlp_sample_block(const nerf_mlp_desc* desc, int32_t precision) {
+  MlpPlan P;
+  if (nerf_plan_from_desc(desc, precision, &P)) return -1;
+  return P.sblk;
+}
 
 extern "C" int32_t nerf_mlp_pack(const nerf_mlp_desc* desc, int32_t precision, const float* flat,
                                  void* packed, void* stream) {

@@ -17,6 +17,7 @@
 // component c = p / L at frequency l = p % L, i.e. reference features
 // 3 + c 2L + 2l + {0: sin, 1: cos} (model.py:323-330).
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define NERF_MAX_LAYERS 16
@@ -124,7 +125,23 @@ struct MlpPlan {
   // then one word per quarter for the rgb branch's 8 blocks (element 8 D + q)
   int64_t st_mask;
   int64_t dz_h[NERF_MAX_LAYERS], dz_bott, dz_rbr, dz_sigma, dz_rgb, dz_per_sample;
+  // Samples per block of the stash and dz buffers.  A buffer of M samples with S
+  // elements per sample is cut into blocks of sblk consecutive samples (the last
+  // one may hold fewer); a block [b, b + n) stores its rows chunk-major: the
+  // 8-element chunk c of sample m at element b S + 8 (c n + m - b).  sblk = 1 is
+  // the plain row-major layout; 32 (one MFMA sample block) for the bf16
+  // weight-stationary kernels of the 8x256 NeRF MLP, whose 16-byte row chunks
+  // of 32 consecutive samples are then 512 contiguous bytes.
+  int sblk;
 };
+
+// Element offset, in a blocked stash or dz buffer (MlpPlan::sblk = B, S elements
+// per sample, M samples), of element e of sample m.
+__host__ __device__ inline int64_t blk_elem(int64_t m, int64_t e, int64_t S, int B, int64_t M) {
+  const int64_t b = m - m % B;
+  const int64_t n = M - b < B ? M - b : B;
+  return b * S + 8 * ((e >> 3) * n + (m - b)) + (e & 7);
+}
 
 // Returns 0 on success, -1 if the description is not supported.
 inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPlan* p) {
@@ -135,6 +152,8 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
   MlpPlan& P = *p;
   P = MlpPlan{};
   P.D = D; P.W = W; P.skip = skip; P.LX = LX; P.LD = LD; P.rgb = LD > 0; P.prec = prec;
+  // the bf16 weight-stationary kernels' topology (their launchers check it)
+  P.sblk = (prec == 1 && W == 256 && D == 8 && LX == 10 && LD == 4 && skip >= 0 && skip + 1 < D) ? 32 : 1;
   P.NB = W / 32;
   P.WR = W / 2;
   P.NBR = (P.WR + 31) / 32;

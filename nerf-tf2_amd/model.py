@@ -190,6 +190,8 @@ class NerfMLP:
                                   dtype=torch.uint8, device=self.device)
         self.stash_per_sample = int(lib.nerf_mlp_stash_floats_per_sample(ctypes_ref(self._desc)))
         self.dz_per_sample = int(lib.nerf_mlp_dz_floats_per_sample(ctypes_ref(self._desc)))
+        # samples per block of the stash / dz layout: batch splits of the weight gradients fall on its multiples
+        self.sample_block = int(lib.nerf_mlp_sample_block(ctypes_ref(self._desc), self.prec))
         self.recompute_chunk: Optional[int] = None  # rays per recomputation chunk (None: stash all)
         self.repack()
 

@@ -4,7 +4,9 @@ stash and dz of one BF16 forward + dX chain, widened exactly to float32, give th
 FP32 kernels the very products the BF16 kernels form (bf16 x bf16 is exact in
 float32), so the two gradients differ only in the float32 summation order.  Both
 precisions share one stash / dz column layout; the FP32 kernels read only the
-columns of each layer's operands."""
+columns of each layer's operands.  The BF16 buffers of the reference net are
+blocked by 32 samples (nerf_mlp_sample_block); they are turned back into plain
+rows for the FP32 kernels, which also pins that layout."""
 import ctypes
 
 import pytest
@@ -13,6 +15,22 @@ import torch
 from test_gpu_parity import DEV
 
 pytestmark = pytest.mark.gpu
+
+
+def unblock(t, B):
+    """Plain rows of a stash / dz buffer (M, S) blocked by B samples: a block of n
+    rows starting at sample b holds the 8-element chunk c of sample m at element
+    b S + 8 (c n + m - b); the last block may hold fewer than B rows."""
+    M, S = t.shape
+    flat = t.reshape(-1)
+    nfull = M // B
+    parts = []
+    if nfull:
+        parts.append(flat[: nfull * B * S].view(nfull, S // 8, B, 8).transpose(1, 2).reshape(nfull * B, S))
+    rem = M - nfull * B
+    if rem:
+        parts.append(flat[nfull * B * S:].view(S // 8, rem, 8).transpose(0, 1).reshape(rem, S))
+    return torch.cat(parts)
 
 
 @pytest.mark.parametrize("net", ["nerf", "proposal"])
@@ -47,9 +65,11 @@ def test_bf16_weight_grads_match_fp32_kernels(nerf, net, M):
         return out.double().cpu()
 
     gb = weight_grads(bf, stash, dz)
+    B = bf.sample_block
+    assert B == (32 if net == "nerf" else 1) and f32.sample_block == 1
     # the ReLU bit words in the bf16 stash are not numbers; no kernel reads them as operands
-    gf = weight_grads(f32, torch.nan_to_num(stash.float(), nan=0.0, posinf=0.0, neginf=0.0).contiguous(),
-                      dz.float().contiguous())
+    gf = weight_grads(f32, torch.nan_to_num(unblock(stash, B).float(), nan=0.0, posinf=0.0, neginf=0.0).contiguous(),
+                      unblock(dz, B).float().contiguous())
     assert bool(torch.isfinite(gb).all()) and bool(torch.isfinite(gf).all())
     assert float(gf.norm()) > 0
     err = float((gb - gf).norm() / gf.norm())
