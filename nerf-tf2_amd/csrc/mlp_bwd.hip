@@ -415,11 +415,77 @@ __device__ __forceinline__ uint32_t blk_chunk_off(uint32_t m, int64_t e, int64_t
   return (uint32_t)((b * ld + 8 * ((e >> 3) * n + (m - b))) * 2);
 }
 
+// Staging of blocked buffers (sample block 32): a wave owns CW 8-feature chunks of
+// the tile for the whole launch; one load covers 64 / CW consecutive rows of each
+// (runs of 128-512 contiguous bytes), lane l row l % (64 / CW) of chunk l / (64 / CW).
+// Lanes l and l ^ 1 then hold rows 2q and 2q + 1 of one chunk: they trade halves
+// (DPP), and each writes the sample pairs {2q, 2q + 1} of four features (the even
+// lane features 0-3, the odd lane 4-7) into the transposed tile.
+__device__ __forceinline__ uint32_t dpp_swap1(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+}
+// d: this lane's 16 bytes (8 bf16 features of its row); result: bf16x2 dwords
+// {row 2q, row 2q + 1} of features 4 odd + i, i = 0..3.  LANE_SEL: lane-dependent
+// byte selectors put the even row's half low (own bytes first on even lanes, the
+// partner's on odd; fewer VALU, two more registers); otherwise the even and odd
+// rows' dwords are selected first and the selectors are constants.
+template <bool LANE_SEL>
+__device__ __forceinline__ u32x4_ pair_rows(u32x4_ d, int odd) {
+  const uint32_t d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3];  // copied out of the vector first
+  const uint32_t x0 = dpp_swap1(odd ? d0 : d2), x1 = dpp_swap1(odd ? d1 : d3);  // the partner's half
+  const uint32_t o0 = odd ? d2 : d0, o1 = odd ? d3 : d1;                        // this lane's half
+  u32x4_ r;
+  if constexpr (LANE_SEL) {
+    const uint32_t slo = odd ? 0x01000504u : 0x05040100u, shi = odd ? 0x03020706u : 0x07060302u;
+    r[0] = __builtin_amdgcn_perm(x0, o0, slo);  // low halves: feature 4 odd + 0
+    r[1] = __builtin_amdgcn_perm(x0, o0, shi);  // high halves: + 1
+    r[2] = __builtin_amdgcn_perm(x1, o1, slo);
+    r[3] = __builtin_amdgcn_perm(x1, o1, shi);
+  } else {
+    const uint32_t e0 = odd ? x0 : o0, e1 = odd ? x1 : o1;  // the even row's dwords
+    const uint32_t f0 = odd ? o0 : x0, f1 = odd ? o1 : x1;  // the odd row's
+    r[0] = __builtin_amdgcn_perm(f0, e0, 0x05040100u);
+    r[1] = __builtin_amdgcn_perm(f0, e0, 0x07060302u);
+    r[2] = __builtin_amdgcn_perm(f1, e1, 0x05040100u);
+    r[3] = __builtin_amdgcn_perm(f1, e1, 0x07060302u);
+  }
+  return r;
+}
+
+// blk_chunk_off in 32-bit arithmetic (a split's buffer stays under 2^31 bytes),
+// for chunk index ch = e / 8 and 32-sample blocks.
+__device__ __forceinline__ uint32_t blk32_chunk_off(uint32_t m, uint32_t ch, uint32_t ld, uint32_t rows) {
+  const uint32_t b = m & ~31u;
+  const uint32_t n = rows - b < 32u ? rows - b : 32u;
+  return m < rows ? (b * ld + 8u * (ch * n + (m - b))) * 2u : 0x80000000u;
+}
+
+// blk_chunk_off for tile-chunks that end inside the split's whole blocks: lane
+// base ((e / 8) 32 + ri) 16 (chunk at element e, row ri), tile-chunk c (64 rows,
+// two blocks of 32), load k (rows k RI .. k RI + RI - 1, inside one block as RI
+// divides 32); ld elements per sample.
+template <int RI>
+__device__ __forceinline__ uint32_t blk_fast_off(uint32_t lane_base, int c, int k, int64_t ld) {
+  const uint32_t blk_bytes = (uint32_t)(64 * ld);  // 32 rows of ld bf16
+  return lane_base + (uint32_t)(2 * c + ((k * RI) >> 5)) * blk_bytes + (uint32_t)(((k * RI) & 31) * 16);
+}
+// The 8 bf16 features of d as floats, added to s.
+__device__ __forceinline__ void add_bf16x8(float (&s)[8], u32x4_ d) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t w = d[k];
+    s[2 * k] += __builtin_bit_cast(float, w << 16);
+    s[2 * k + 1] += __builtin_bit_cast(float, w & 0xFFFF0000u);
+  }
+}
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const __bf16* base, int64_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(base), 0, (int)bytes, 0x00020000);
 }
 
-template <int KB, int NB>
+// BLK: the operands are blocked by 32 samples and staged as described at
+// pair_rows (wave w owns NFG / 4 chunks of each operand); otherwise plain rows.
+template <int KB, int NB, bool BLK>
 __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   constexpr bool kModeN = NB >= 4;
   constexpr int RB = kModeN ? KB : (KB >= 8 ? 2 : 1);  // row blocks per wave
@@ -433,6 +499,8 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   constexpr int NFGX = KB >= 8 ? 32 : KB >= 4 ? 16 : 8, NFGZ = NB >= 8 ? 32 : NB >= 4 ? 16 : 8;
   constexpr int RX = NFGX / 8, RZ = NFGZ / 8;
   constexpr int PPX = 256 / NFGX, PPZ = 256 / NFGZ;  // sample pairs per round
+  constexpr int CWX = NFGX / 4, CWZ = NFGZ / 4;      // blocked: chunks per wave
+  constexpr int RIX = 64 / CWX, RIZ = 64 / CWZ;      // blocked: rows per load
   typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kWF][kWP];
   __shared__ __attribute__((aligned(16))) __bf16 Zt[2][kWF][kWP];
@@ -444,7 +512,9 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   const int t = threadIdx.x, lane = t & 63, r = lane & 31, q = lane >> 5;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int fgx = t % NFGX, psx = t / NFGX, fgz = t % NFGZ, psz = t / NFGZ;
-  const int swx = 4 * ((fgx >> 1) & 7), swz = 4 * ((fgz >> 1) & 7);
+  // blocked staging: this lane's chunk (fixed) and row of each operand
+  const int chx = wv * CWX + lane / RIX, rix = lane % RIX, chz = wv * CWZ + lane / RIZ, riz = lane % RIZ;
+  const int swx = 4 * (((BLK ? chx : fgx) >> 1) & 7), swz = 4 * (((BLK ? chz : fgz) >> 1) & 7);
   const int qx = q ^ (r >> 4);  // operand reads: swizzled 8-sample group of this lane
   const __amdgpu_buffer_rsrc_t rx = rows_rsrc(a.X + mb * a.ldx, rows * a.ldx * 2);
   const __amdgpu_buffer_rsrc_t rz = rows_rsrc(a.Z + mb * a.ldz, rows * a.ldz * 2);
@@ -460,8 +530,32 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
 
-  u32x4_ xr[RX][2], zr[RZ][2];
+  u32x4_ xr[RX][2], zr[RZ][2];  // blocked: load k of CW = 2 R in xr[k / 2][k % 2]
+  const uint32_t bxf = (uint32_t)((((J.xoff >> 3) + chx) * 32 + rix) * 16);  // blocked fast-path lane bases
+  const uint32_t bzf = (uint32_t)((((J.zoff >> 3) + chz) * 32 + riz) * 16);
   auto load_chunk = [&](int c) {
+    if constexpr (BLK) {
+      if ((int64_t)(c + 1) * kWC <= (rows & ~(int64_t)31)) {  // whole blocks (wave-uniform)
+#pragma unroll
+        for (int k = 0; k < CWX; ++k)
+          xr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, 0);
+#pragma unroll
+        for (int k = 0; k < CWZ; ++k)
+          zr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, 0);
+      } else {
+#pragma unroll
+        for (int k = 0; k < CWX; ++k)
+          xr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(
+              rx, blk_chunk_off((uint32_t)(c * kWC + k * RIX + rix), J.xoff + 8 * chx, a.ldx, rows, a.sb_shift), 0,
+              0);
+#pragma unroll
+        for (int k = 0; k < CWZ; ++k)
+          zr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(
+              rz, blk_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), J.zoff + 8 * chz, a.ldz, rows, a.sb_shift), 0,
+              0);
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < RX; ++i)
 #pragma unroll
@@ -480,6 +574,30 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
       }
   };
   auto store_chunk = [&](int buf) {
+    if constexpr (BLK) {
+#pragma unroll
+      for (int k = 0; k < CWX; ++k) {
+        const int col = 2 * (((k * RIX + rix) >> 1) ^ swx);
+        const u32x4_ px = pair_rows<true>(xr[k / 2][k % 2], rix & 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t v = px[i];
+          *reinterpret_cast<uint32_t*>(&Xt[buf][8 * chx + 4 * (rix & 1) + i][col]) = v;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CWZ; ++k) {
+        const int col = 2 * (((k * RIZ + riz) >> 1) ^ swz);
+        const u32x4_ pz = pair_rows<true>(zr[k / 2][k % 2], riz & 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t v = pz[i];
+          *reinterpret_cast<uint32_t*>(&Zt[buf][8 * chz + 4 * (riz & 1) + i][col]) = v;
+        }
+        add_bf16x8(bsum, zr[k / 2][k % 2]);
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < RX; ++i) {
       const bf16x8 v0 = __builtin_bit_cast(bf16x8, xr[i][0]), v1 = __builtin_bit_cast(bf16x8, xr[i][1]);
@@ -546,7 +664,21 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
       }
     }
   }
-  if (J.bpart) {
+  if (J.bpart && BLK) {
+    // the RIZ lanes of a chunk (all in this wave), in a fixed order
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+      for (int msk = 1; msk < RIZ; msk <<= 1) bsum[e] += shfl_xor(bsum[e], msk);
+    }
+    if (riz == 0) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int n = 8 * chz + e;
+        if (n < J.N) J.bpart[(int64_t)split * J.N + n] = bsum[e];
+      }
+    }
+  } else if (J.bpart) {
     // the stores above never touched LDS; the loop ended with a barrier, so Xt is free:
     // lanes with the same feature group are NFGZ apart (within a wave, then across waves)
 #pragma unroll
@@ -577,10 +709,12 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
 // (128 accumulators), so a k-step reads 6 operands from LDS for 8 MFMAs, and
 // registers remain for two chunks of loads in flight per thread (chunk k is
 // loaded into register set k % 2 two iterations before it goes to LDS).  Same
-// chunking, transposed XOR-swizzled LDS tiles and fixed-order reductions as
-// wgrad_bf16_kernel.
+// chunking and transposed XOR-swizzled LDS tiles as wgrad_bf16_kernel; its
+// operands are always the blocked buffers of the 8x256 net in bf16 (sample block
+// 32), staged as described at pair_rows: wave w owns feature chunks 4 w .. 4 w +
+// 3, a load covers 16 rows of each.
 __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
-  constexpr int NFG = 32, R = 2, PP = 512 / NFG;  // feature groups, rounds, sample pairs per round
+  constexpr int NFG = 32, CW = NFG / 8, RI = 64 / CW;  // feature chunks, chunks per wave, rows per load
   typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kWF][kWP];
   __shared__ __attribute__((aligned(16))) __bf16 Zt[2][kWF][kWP];
@@ -591,8 +725,8 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
   const int64_t rows = me > mb ? me - mb : 0;
   const int t = threadIdx.x, lane = t & 63, r = lane & 31, q = lane >> 5;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int fg = t % NFG, ps = t / NFG;
-  const int sw = 4 * ((fg >> 1) & 7);
+  const int chunk = wv * CW + lane / RI, rin = lane % RI, odd = rin & 1;  // this lane's chunk (fixed) and row
+  const int sw = 4 * ((chunk >> 1) & 7);
   const int qx = q ^ (r >> 4);
   const __amdgpu_buffer_rsrc_t rx = rows_rsrc(a.X + mb * a.ldx, rows * a.ldx * 2);
   const __amdgpu_buffer_rsrc_t rz = rows_rsrc(a.Z + mb * a.ldz, rows * a.ldz * 2);
@@ -606,35 +740,33 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
 
-  u32x4_ xr[2][R][2], zr[2][R][2];
+  u32x4_ xr[2][CW], zr[2][CW];
+  const uint32_t chx32 = (uint32_t)((J.xoff >> 3) + chunk), chz32 = (uint32_t)((J.zoff >> 3) + chunk);
+  const uint32_t ldx32 = (uint32_t)a.ldx, ldz32 = (uint32_t)a.ldz, rows32 = (uint32_t)rows;
   // per-load byte offsets in the VGPR offset, which the buffer range check covers
   // (rows past the split read zeros), not in the unchecked scalar offset
   auto load_chunk = [&](auto set, int c) {
     constexpr int S = decltype(set)::value;
 #pragma unroll
-    for (int i = 0; i < R; ++i)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t row = (uint32_t)(c * kWC + 2 * PP * i + 2 * ps + h);
-        xr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(
-            rx, blk_chunk_off(row, J.xoff + 8 * fg, a.ldx, rows, a.sb_shift), 0, 0);
-        zr[S][i][h] = __builtin_amdgcn_raw_buffer_load_b128(
-            rz, blk_chunk_off(row, J.zoff + 8 * fg, a.ldz, rows, a.sb_shift), 0, 0);
-      }
+    for (int k = 0; k < CW; ++k) {
+      const uint32_t row = (uint32_t)(c * kWC + k * RI + rin);
+      xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk32_chunk_off(row, chx32, ldx32, rows32), 0, 0);
+      zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk32_chunk_off(row, chz32, ldz32, rows32), 0, 0);
+    }
   };
   auto store_chunk = [&](auto set, int buf) {
     constexpr int S = decltype(set)::value;
 #pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const int col = 2 * ((ps + PP * i) ^ sw);
-      const bf16x8 x0 = __builtin_bit_cast(bf16x8, xr[S][i][0]), x1 = __builtin_bit_cast(bf16x8, xr[S][i][1]);
-      const bf16x8 z0 = __builtin_bit_cast(bf16x8, zr[S][i][0]), z1 = __builtin_bit_cast(bf16x8, zr[S][i][1]);
+    for (int k = 0; k < CW; ++k) {
+      const int col = 2 * (((k * RI + rin) >> 1) ^ sw);  // the swizzled position of this lane's sample pair
+      const u32x4_ px = pair_rows<false>(xr[S][k], odd), pz = pair_rows<false>(zr[S][k], odd);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        *reinterpret_cast<bf16x2*>(&Xt[buf][8 * fg + e][col]) = bf16x2{x0[e], x1[e]};
-        *reinterpret_cast<bf16x2*>(&Zt[buf][8 * fg + e][col]) = bf16x2{z0[e], z1[e]};
-        bsum[e] += (float)z0[e] + (float)z1[e];
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t vx = px[i], vz = pz[i];
+        *reinterpret_cast<uint32_t*>(&Xt[buf][8 * chunk + 4 * odd + i][col]) = vx;
+        *reinterpret_cast<uint32_t*>(&Zt[buf][8 * chunk + 4 * odd + i][col]) = vz;
       }
+      add_bf16x8(bsum, zr[S][k]);
     }
   };
   auto operand = [&](const __bf16 (*tile)[kWP], int blk, int s) {
@@ -700,23 +832,17 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
     }
   }
   if (J.bpart) {
-    // lanes l and l + 32 hold the same feature group; then the 8 waves through LDS
+    // the RI lanes of a chunk (all in this wave), in a fixed order
 #pragma unroll
-    for (int e = 0; e < 8; ++e) bsum[e] += shfl_xor(bsum[e], 32);
-    float* red = reinterpret_cast<float*>(&Xt[0][0][0]);  // [8 waves][32][8]
-    if (lane < NFG) {
+    for (int e = 0; e < 8; ++e) {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) red[(wv * NFG + lane) * 8 + e] = bsum[e];
+      for (int msk = 1; msk < RI; msk <<= 1) bsum[e] += shfl_xor(bsum[e], msk);
     }
-    __syncthreads();
-    if (wv == 0 && lane < NFG) {
+    if (rin == 0) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        float v = 0.f;
-#pragma unroll
-        for (int w = 0; w < 8; ++w) v += red[(w * NFG + lane) * 8 + e];
-        const int n = 8 * lane + e;
-        if (n < J.N) J.bpart[(int64_t)split * J.N + n] = v;
+        const int n = 8 * chunk + e;
+        if (n < J.N) J.bpart[(int64_t)split * J.N + n] = bsum[e];
       }
     }
   }
@@ -727,12 +853,21 @@ int wg16_blocks(int n) { return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8; }
 // All jobs of `a` share the tile shape (kb, nb) = wg16_blocks of their 32-feature blocks.
 int launch_wgrad_bf16(const Wgrad16Args& a, int kb, int nb, hipStream_t s) {
   const unsigned grid = (unsigned)(a.splits * a.njobs);
+  const bool blk = a.sb_shift == 5;
+  if (blk != (a.sb_shift != 0)) return -1;  // sample blocks of 1 or 32 only
   if (kb == 8 && nb == 8) {
+    if (!blk) return -1;  // only the blocked buffers of the 8x256 net have 256 x 256 layers in bf16
     wgrad_bf16_w8_kernel<<<grid, 512, 0, s>>>(a);
     return 0;
   }
-#define NERF_WG16(KB_, NB_) \
-  if (kb == KB_ && nb == NB_) { wgrad_bf16_kernel<KB_, NB_><<<grid, 256, 0, s>>>(a); return 0; }
+#define NERF_WG16(KB_, NB_)                                                             \
+  if (kb == KB_ && nb == NB_) {                                                         \
+    if (blk)                                                                            \
+      wgrad_bf16_kernel<KB_, NB_, true><<<grid, 256, 0, s>>>(a);                        \
+    else                                                                                \
+      wgrad_bf16_kernel<KB_, NB_, false><<<grid, 256, 0, s>>>(a);                       \
+    return 0;                                                                           \
+  }
 #define NERF_WG16_ROW(KB_) NERF_WG16(KB_, 1) NERF_WG16(KB_, 2) NERF_WG16(KB_, 4) NERF_WG16(KB_, 8)
   NERF_WG16_ROW(1) NERF_WG16_ROW(2) NERF_WG16_ROW(4)
   NERF_WG16(8, 1) NERF_WG16(8, 2) NERF_WG16(8, 4)  // (8, 8): wgrad_bf16_w8_kernel above
@@ -1105,7 +1240,8 @@ static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const v
         b.job[i].bpart = J.bias ? wp : nullptr;
         if (J.bias) wp += sg * J.N;
       }
-      launch_wgrad_bf16(b, G.kb, G.nb, s);
+      NERF_CHECK_ARG(launch_wgrad_bf16(b, G.kb, G.nb, s) == 0,
+                     "nerf_mlp_weight_grads: no bf16 weight GEMM for this tile shape and stash layout");
       ReduceBatch rbt;
       rbt.njobs = n;
       rbt.first[0] = 0;
