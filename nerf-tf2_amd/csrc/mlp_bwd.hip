@@ -286,40 +286,51 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
 #pragma unroll
   for (int e = 0; e < 16; ++e) bs[e] = 0.f;
   const bool do_bias = a.bpart && tk == 0;
-  // loader mapping: 256 threads x 16 floats = 32 rows x 128 cols
+  // loader mapping: 256 threads x 16 floats = 32 rows x 128 cols; the next chunk is
+  // loaded into registers while the current one is multiplied
   const int lr = tid >> 3, lc = (tid & 7) * 16;
-  for (int64_t c0 = mb; c0 < me; c0 += kChunk) {
+  floatx4 xv[4], zv[4];
+  auto load = [&](int64_t c0) {
     const int64_t mrow = c0 + lr;
     const bool rok = mrow < me;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int col = lc + 4 * v;
-      floatx4 xv = {0.f, 0.f, 0.f, 0.f}, zv = {0.f, 0.f, 0.f, 0.f};
+      xv[v] = floatx4{0.f, 0.f, 0.f, 0.f};
+      zv[v] = floatx4{0.f, 0.f, 0.f, 0.f};
       if (rok) {
         const int64_t xo = mrow * a.ldx + a.xoff + k0 + col;
         if (k0 + col + 3 < a.K) {
-          xv = *reinterpret_cast<const floatx4*>(a.X + xo);
+          xv[v] = *reinterpret_cast<const floatx4*>(a.X + xo);
         } else {
 #pragma unroll
           for (int e = 0; e < 4; ++e)
-            if (k0 + col + e < a.K) xv[e] = a.X[xo + e];
+            if (k0 + col + e < a.K) xv[v][e] = a.X[xo + e];
         }
         if (n0 + col + 3 < a.N) {
-          zv = *reinterpret_cast<const floatx4*>(a.Z + mrow * a.ldz + a.zoff + n0 + col);
+          zv[v] = *reinterpret_cast<const floatx4*>(a.Z + mrow * a.ldz + a.zoff + n0 + col);
         } else {
 #pragma unroll
           for (int e = 0; e < 4; ++e)
-            if (n0 + col + e < a.N) zv[e] = a.Z[mrow * a.ldz + a.zoff + n0 + col + e];
+            if (n0 + col + e < a.N) zv[v][e] = a.Z[mrow * a.ldz + a.zoff + n0 + col + e];
         }
       }
+    }
+  };
+  if (mb < me) load(mb);
+  for (int64_t c0 = mb; c0 < me; c0 += kChunk) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int col = lc + 4 * v;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        Xs[lr][col + e] = xv[e];
-        Zs[lr][col + e] = zv[e];
-        bs[4 * v + e] += zv[e];
+        Xs[lr][col + e] = xv[v][e];
+        Zs[lr][col + e] = zv[v][e];
+        bs[4 * v + e] += zv[v][e];
       }
     }
     __syncthreads();
+    if (c0 + kChunk < me) load(c0 + kChunk);  // wave-uniform
 #pragma unroll 4
     for (int s = 0; s < kChunk / 2; ++s) {
       const int row = 2 * s + q;
@@ -1185,7 +1196,6 @@ static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const v
   WgradJob jobs[2 * NERF_MAX_LAYERS];
   const int nj = wgrad_jobs(P, jobs);
   const int64_t splits = splits_for(M);
-  const int64_t mps = (M + splits - 1) / splits;
   auto reduce_args = [&](const WgradJob& J, const float* jp, const float* jb, int nsplit) {
     ReduceArgs r;
     r.part = jp;
@@ -1264,17 +1274,20 @@ static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const v
       a.ldz = P.dz_per_sample;
       a.zoff = J.zoff;
       a.M = M;
-      a.m_per_split = mps;
       a.K = J.K;
       a.N = J.N;
       a.tiles_k = (J.K + kTile - 1) / kTile;
       a.tiles_n = (J.N + kTile - 1) / kTile;
-      a.splits = (int)splits;
+      // three workgroups fit per CU: a four-tile layer takes 3/4 of the splits, so that
+      // its 4 x 192 workgroups run as one round on 256 CUs
+      const int64_t sj = a.tiles_k * a.tiles_n >= 4 && splits >= 4 ? splits * 3 / 4 : splits;
+      a.m_per_split = (M + sj - 1) / sj;
+      a.splits = (int)sj;
       a.part = part;
       a.bpart = bpart;
-      const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * ((splits + 7) / 8) * 8;
+      const int64_t nb = (int64_t)a.tiles_k * a.tiles_n * ((sj + 7) / 8) * 8;
       wgrad_gemm_kernel<<<(unsigned)nb, 256, 0, s>>>(a);
-      wgrad_reduce_kernel<<<(unsigned)reduce_blocks(J), 256, 0, s>>>(reduce_args(J, part, bpart, (int)splits));
+      wgrad_reduce_kernel<<<(unsigned)reduce_blocks(J), 256, 0, s>>>(reduce_args(J, part, bpart, (int)sj));
     }
   }
   NERF_LAUNCH_CHECK("nerf_mlp_weight_grads");
