@@ -17,6 +17,7 @@
 #include "common.h"
 #include "mlp_kernels.h"
 #include "mlp_bwd_bf16ws.h"
+#include "mlp_bwd_bf16ws8.h"
 #include "mlp_bwd_f32r.h"
 
 namespace {
@@ -1062,7 +1063,7 @@ int launch_bwd_bf16_any(const MlpPlan& P, const void* packed, int64_t M, const f
   // the stash / dz layout (MlpPlan::sblk) must be the one the kernel reads and writes
   if constexpr (W == 256 && LX == 10 && LD == 4 && SKIP >= 0 && SKIP + 1 < D)
     return P.sblk != 32 ? -1
-                        : launch_bwd_bf16ws<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
+                        : launch_bwd_bf16ws8<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
   else
     return P.sblk != 1 ? -1
                        : launch_bwd_bf16<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);

@@ -1,9 +1,9 @@
 // Building blocks of the weight-stationary BF16 kernels of the reference
 // topology (W = 256): the forward (mlp_fwd_bf16ws8.h) and the dX chain
-// (mlp_bwd_bf16ws.h).
+// (mlp_bwd_bf16ws8.h), eight waves per workgroup, wave w owning output block w.
 //
 // A workgroup carries 128 samples (4 blocks of 32) through the MLP.  Each wave
-// owns some 32-feature output blocks of every layer and holds that slice of the
+// owns a 32-feature output block of every layer and holds that slice of the
 // packed bf16 weights in VGPRs, split into two halves of the layer's k-steps
 // (wlo, whi).  A layer runs k-half by k-half over all four sample blocks, with
 // the accumulators of all four blocks live: once the first half has been used,
@@ -13,7 +13,7 @@
 // L2 feeds 128 samples, and no more than one layer's slice is resident.
 //
 // B operands (activations, bf16, in the MFMA B layout) stream from a double-
-// buffered LDS tile through a ring of registers (run_steps).  Accumulators can
+// buffered LDS tile through a ring of registers (ws8::run_steps).  Accumulators can
 // start from the bias (the first MFMA's C operand is the bias in accumulator
 // layout, bias_acc), and an epilogue converts a block to bf16 (acc_to_bf16),
 // applies ReLU as a packed integer max on the bf16 bits (relu_bf16: equal to
@@ -54,17 +54,6 @@ __device__ __forceinline__ bf16x8 ldw(__amdgpu_buffer_rsrc_t r, uint32_t voff, i
 
 __device__ __forceinline__ floatx4 ldf4(__amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
   return __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, (int)soff, 0));
-}
-
-// k-steps OFF .. OFF+N-1 of output blocks nb0, nb0 + 1 of the packed layer at
-// byte offset p_layer into w[0 .. N); lane_off = 16 * lane.
-template <int N, int OFF>
-__device__ __forceinline__ void load_w(bf16x8 (&w)[kHalf][2], __amdgpu_buffer_rsrc_t r, int64_t p_layer, int NBo,
-                                       int nb0, uint32_t lane_off) {
-#pragma unroll
-  for (int t = 0; t < N; ++t)
-#pragma unroll
-    for (int i = 0; i < 2; ++i) w[t][i] = ldw(r, lane_off, p_layer + (int64_t)((OFF + t) * NBo + nb0 + i) * 1024);
 }
 
 // Bias of output block nb in accumulator layout: element 4a + r of lane
@@ -139,41 +128,6 @@ __device__ __forceinline__ lds_b8* lds_lane(const bf16x8* arr, int off, int lane
 // `in` for t < 16, the xyz encoding for t >= 16 (per-lane pointers).
 __device__ __forceinline__ bf16x8 b_operand(const lds_b8* in, const lds_b8* ex, int sb, int t) {
   return t < kT ? in[sb * kSlab + t * 64] : ex[(sb * kTX + (t - kT)) * 64];
-}
-
-// acc[sb][i] (+)= sum over k-steps T0 .. T0+H-1 of w[k - T0][i] * B(sb, k), for
-// all four sample blocks.  INIT: the first k-step starts from the bias cb.
-// After the last k-step of each block, epi(sb) runs (no-op for a first half).
-struct NoEpilogue {
-  __device__ void operator()(int) const {}
-};
-
-// Without an epilogue (a first half) step s runs sample block pair p = s / 2H,
-// k-step (s / 2) % H, block 2p + s % 2: the two blocks of a pair alternate, so
-// consecutive MFMAs go to four independent accumulators.  With an epilogue the
-// blocks run one after the other, so each block's epilogue overlaps the next
-// block's MFMAs.
-template <int T0, int H, bool INIT, typename Epi>
-__device__ __forceinline__ void run_steps(floatx16 (&acc)[kBlocks][2], const bf16x8 (&w)[kHalf][2],
-                                          const floatx16 (&cb)[2], const lds_b8* in, const lds_b8* ex,
-                                          Epi&& epi) {
-  constexpr int S = kBlocks * H;
-  constexpr bool PAIRS = std::is_same_v<std::decay_t<Epi>, NoEpilogue>;
-  auto blk = [](int s) { return PAIRS ? 2 * (s / (2 * H)) + (s & 1) : s / H; };
-  auto kst = [](int s) { return PAIRS ? (s >> 1) % H : s % H; };
-  constexpr int R = kRing;
-  bf16x8 b[R];
-#pragma unroll
-  for (int s = 0; s < R - 1; ++s) b[s] = b_operand(in, ex, blk(s), T0 + kst(s));
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (s + R - 1 < S) b[(s + R - 1) % R] = b_operand(in, ex, blk(s + R - 1), T0 + kst(s + R - 1));
-    const int sb = blk(s), k = kst(s);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-      acc[sb][i] = mfma_bf16(w[k][i], b[s % R], (INIT && k == 0) ? cb[i] : acc[sb][i]);
-    if (!PAIRS && k == H - 1) epi(sb);
-  }
 }
 
 template <int A, int B, typename F>
