@@ -136,12 +136,13 @@ __global__ __launch_bounds__(256) void mlp_bwd_f32_kernel(
 }
 
 template <int NBo>
-__device__ __forceinline__ void relu_mask_rows_bf16(floatx16 (&acc)[NBo], const __bf16* row, int g) {
+__device__ __forceinline__ void relu_mask_rows_bf16(floatx16 (&acc)[NBo], BlkRow<const __bf16> row, int64_t off,
+                                                    int g) {
 #pragma unroll
   for (int nb = 0; nb < NBo; ++nb)
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const bf16x4 h = *reinterpret_cast<const bf16x4*>(row + 32 * nb + 8 * a + 4 * g);
+      const bf16x4 h = *reinterpret_cast<const bf16x4*>(row.at(off + 32 * nb + 8 * a + 4 * g));
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[nb][4 * a + r] = (float)h[r] > 0.f ? acc[nb][4 * a + r] : 0.f;
     }
@@ -165,8 +166,11 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
   const int64_t m = blk * 32 + j;
   const bool valid = m < M;
   const int64_t mm = valid ? m : M - 1;
-  const __bf16* srow = stash + mm * P.stash_per_sample;
-  __bf16* zrow = dz + mm * P.dz_per_sample;
+  // this sample's rows of the stash and dz, blocked by 32 samples (the wave's block;
+  // lanes past M read the block's last row)
+  const int nrow = (int)(M - blk * 32 < 32 ? M - blk * 32 : 32);
+  const BlkRow<const __bf16> srow{stash + blk * 32 * P.stash_per_sample, nrow, (int)(mm - blk * 32)};
+  const BlkRow<__bf16> zrow{dz + blk * 32 * P.dz_per_sample, nrow, (int)(mm - blk * 32)};
   const float sg = sigma[mm];
   const float ds = valid ? d_sigma[mm] : 0.f;
   const float dzs = C::RGB ? (sg > 0.f ? ds : 0.f) : ds * (1.f - expf(-sg));
@@ -188,13 +192,13 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
         const floatx4 w = hr4[16 * nb + i];
         accr[nb][i] = dzr[0] * w[0] + dzr[1] * w[1] + dzr[2] * w[2];
       }
-    relu_mask_rows_bf16(accr, srow + P.st_rbr, g);
+    relu_mask_rows_bf16(accr, srow, P.st_rbr, g);
     if (valid) {
-      store_rows_bf16(accr, zrow + P.dz_rbr, g);
+      store_rows_bf16(accr, zrow, P.dz_rbr, g);
       if (g == 0) {
-        zrow[P.dz_rgb + 0] = (__bf16)dzr[0];
-        zrow[P.dz_rgb + 1] = (__bf16)dzr[1];
-        zrow[P.dz_rgb + 2] = (__bf16)dzr[2];
+        *zrow.at(P.dz_rgb + 0) = (__bf16)dzr[0];
+        *zrow.at(P.dz_rgb + 1) = (__bf16)dzr[1];
+        *zrow.at(P.dz_rgb + 2) = (__bf16)dzr[2];
       }
     }
     wave_sync();
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
     wave_sync();
     mfma_lds_bf16<C::NB, 2 * C::NBR>(acc, reinterpret_cast<const bf16x8*>(pk + P.p_bwd[D + 1]), act,
                                      lane);
-    if (valid) store_rows_bf16(acc, zrow + P.dz_bott, g);
+    if (valid) store_rows_bf16(acc, zrow, P.dz_bott, g);
     wave_sync();
     store_act_bf16(acc, act, lane);
     zero_acc(acc);
@@ -223,9 +227,9 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
         for (int r = 0; r < 4; ++r) acc[nb][4 * a + r] = acc[nb][4 * a + r] + w[r] * dzs;
       }
   }
-  if (valid && g == 0) zrow[P.dz_sigma] = (__bf16)dzs;
-  relu_mask_rows_bf16(acc, srow + P.st_h[D - 1], g);
-  if (valid) store_rows_bf16(acc, zrow + P.dz_h[D - 1], g);
+  if (valid && g == 0) *zrow.at(P.dz_sigma) = (__bf16)dzs;
+  relu_mask_rows_bf16(acc, srow, P.st_h[D - 1], g);
+  if (valid) store_rows_bf16(acc, zrow, P.dz_h[D - 1], g);
 #pragma unroll 1
   for (int l = D - 1; l >= 1; --l) {
     wave_sync();
@@ -233,8 +237,8 @@ __global__ __launch_bounds__(256) void mlp_bwd_bf16_kernel(
     zero_acc(acc);
     wave_sync();
     mfma_lds_bf16<C::NB, TH>(acc, reinterpret_cast<const bf16x8*>(pk + P.p_bwd[l]), act, lane);
-    relu_mask_rows_bf16(acc, srow + P.st_h[l - 1], g);
-    if (valid) store_rows_bf16(acc, zrow + P.dz_h[l - 1], g);
+    relu_mask_rows_bf16(acc, srow, P.st_h[l - 1], g);
+    if (valid) store_rows_bf16(acc, zrow, P.dz_h[l - 1], g);
   }
 }
 
@@ -415,18 +419,6 @@ struct Wgrad16Args {
   Wgrad16Job job[kMaxJobs];
 };
 
-// Byte offset of the 8-element chunk at element e (a multiple of 8) of row m of a
-// split (rows [0, rows) from its first sample; splits start on sample-block
-// boundaries) in a blocked buffer with ld elements per sample; rows past the split
-// get an offset outside its buffer resource (they read zeros).
-__device__ __forceinline__ uint32_t blk_chunk_off(uint32_t m, int64_t e, int64_t ld, int64_t rows, int sh) {
-  if (m >= rows) return 0x80000000u;
-  const uint32_t b = (m >> sh) << sh;
-  const int64_t left = rows - b;
-  const int64_t n = left < (1 << sh) ? left : (1 << sh);
-  return (uint32_t)((b * ld + 8 * ((e >> 3) * n + (m - b))) * 2);
-}
-
 // Staging of blocked buffers (sample block 32): a wave owns CW 8-feature chunks of
 // the tile for the whole launch; one load covers 64 / CW consecutive rows of each
 // (runs of 128-512 contiguous bytes), lane l row l % (64 / CW) of chunk l / (64 / CW).
@@ -464,8 +456,11 @@ __device__ __forceinline__ u32x4_ pair_rows(u32x4_ d, int odd) {
   return r;
 }
 
-// blk_chunk_off in 32-bit arithmetic (a split's buffer stays under 2^31 bytes),
-// for chunk index ch = e / 8 and 32-sample blocks.
+// Byte offset of 8-element chunk ch of row m of a split (rows [0, rows) from its
+// first sample; splits start on 32-sample block boundaries) in a buffer blocked
+// by 32 samples with ld elements per sample, in 32-bit arithmetic (a split's
+// buffer stays under 2^31 bytes); rows past the split get an offset outside its
+// buffer resource (they read zeros).
 __device__ __forceinline__ uint32_t blk32_chunk_off(uint32_t m, uint32_t ch, uint32_t ld, uint32_t rows) {
   const uint32_t b = m & ~31u;
   const uint32_t n = rows - b < 32u ? rows - b : 32u;
@@ -495,25 +490,19 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const __bf16* base, 
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(base), 0, (int)bytes, 0x00020000);
 }
 
-// BLK: the operands are blocked by 32 samples and staged as described at
-// pair_rows (wave w owns NFG / 4 chunks of each operand); otherwise plain rows.
-template <int KB, int NB, bool BLK>
+// The operands are blocked by 32 samples and staged as described at pair_rows:
+// wave w owns NFG / 4 8-feature chunks of each operand (NFG = the operand's
+// chunks), and the transposed LDS tile stores sample pair p of feature row R at
+// pair position p ^ 4 ((R >> 4) & 7), which keeps every 8-sample MFMA operand
+// contiguous and spreads the pair writes of a wave over the banks.
+template <int KB, int NB>
 __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   constexpr bool kModeN = NB >= 4;
   constexpr int RB = kModeN ? KB : (KB >= 8 ? 2 : 1);  // row blocks per wave
   constexpr int CB = kModeN ? NB / 4 : NB;             // column blocks per wave
-  // Staging: thread t loads 16 bytes (8 features, feature group fg = t % NFG) of two
-  // consecutive sample rows (sample pair ps + (256 / NFG) i, round i < NFG / 8), so
-  // a wave reads whole 16 B x NFG row segments; the transposed LDS tile stores
-  // sample pair p of feature row R at pair position p ^ 4 ((R >> 4) & 7), which
-  // keeps every 8-sample MFMA operand contiguous and spreads the pair writes of a
-  // wave over the banks.
   constexpr int NFGX = KB >= 8 ? 32 : KB >= 4 ? 16 : 8, NFGZ = NB >= 8 ? 32 : NB >= 4 ? 16 : 8;
-  constexpr int RX = NFGX / 8, RZ = NFGZ / 8;
-  constexpr int PPX = 256 / NFGX, PPZ = 256 / NFGZ;  // sample pairs per round
-  constexpr int CWX = NFGX / 4, CWZ = NFGZ / 4;      // blocked: chunks per wave
-  constexpr int RIX = 64 / CWX, RIZ = 64 / CWZ;      // blocked: rows per load
-  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  constexpr int CWX = NFGX / 4, CWZ = NFGZ / 4;  // chunks per wave
+  constexpr int RIX = 64 / CWX, RIZ = 64 / CWZ;  // rows per load
   __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kWF][kWP];
   __shared__ __attribute__((aligned(16))) __bf16 Zt[2][kWF][kWP];
   const int split = blockIdx.x % a.splits;
@@ -523,10 +512,9 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   const int64_t rows = me > mb ? me - mb : 0;
   const int t = threadIdx.x, lane = t & 63, r = lane & 31, q = lane >> 5;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int fgx = t % NFGX, psx = t / NFGX, fgz = t % NFGZ, psz = t / NFGZ;
-  // blocked staging: this lane's chunk (fixed) and row of each operand
+  // this lane's chunk (fixed) and row of each operand
   const int chx = wv * CWX + lane / RIX, rix = lane % RIX, chz = wv * CWZ + lane / RIZ, riz = lane % RIZ;
-  const int swx = 4 * (((BLK ? chx : fgx) >> 1) & 7), swz = 4 * (((BLK ? chz : fgz) >> 1) & 7);
+  const int swx = 4 * ((chx >> 1) & 7), swz = 4 * ((chz >> 1) & 7);
   const int qx = q ^ (r >> 4);  // operand reads: swizzled 8-sample group of this lane
   const __amdgpu_buffer_rsrc_t rx = rows_rsrc(a.X + mb * a.ldx, rows * a.ldx * 2);
   const __amdgpu_buffer_rsrc_t rz = rows_rsrc(a.Z + mb * a.ldz, rows * a.ldz * 2);
@@ -542,90 +530,51 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
 
-  u32x4_ xr[RX][2], zr[RZ][2];  // blocked: load k of CW = 2 R in xr[k / 2][k % 2]
-  const uint32_t bxf = (uint32_t)((((J.xoff >> 3) + chx) * 32 + rix) * 16);  // blocked fast-path lane bases
+  u32x4_ xr[CWX], zr[CWZ];
+  const uint32_t bxf = (uint32_t)((((J.xoff >> 3) + chx) * 32 + rix) * 16);  // fast-path lane bases
   const uint32_t bzf = (uint32_t)((((J.zoff >> 3) + chz) * 32 + riz) * 16);
+  const uint32_t chx32 = (uint32_t)((J.xoff >> 3) + chx), chz32 = (uint32_t)((J.zoff >> 3) + chz);
+  const uint32_t ldx32 = (uint32_t)a.ldx, ldz32 = (uint32_t)a.ldz, rows32 = (uint32_t)rows;
   auto load_chunk = [&](int c) {
-    if constexpr (BLK) {
-      if ((int64_t)(c + 1) * kWC <= (rows & ~(int64_t)31)) {  // whole blocks (wave-uniform)
+    if ((int64_t)(c + 1) * kWC <= (rows & ~(int64_t)31)) {  // whole blocks (wave-uniform)
 #pragma unroll
-        for (int k = 0; k < CWX; ++k)
-          xr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, 0);
+      for (int k = 0; k < CWX; ++k)
+        xr[k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, 0);
 #pragma unroll
-        for (int k = 0; k < CWZ; ++k)
-          zr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, 0);
-      } else {
+      for (int k = 0; k < CWZ; ++k)
+        zr[k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, 0);
+    } else {
 #pragma unroll
-        for (int k = 0; k < CWX; ++k)
-          xr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(
-              rx, blk_chunk_off((uint32_t)(c * kWC + k * RIX + rix), J.xoff + 8 * chx, a.ldx, rows, a.sb_shift), 0,
-              0);
+      for (int k = 0; k < CWX; ++k)
+        xr[k] = __builtin_amdgcn_raw_buffer_load_b128(
+            rx, blk32_chunk_off((uint32_t)(c * kWC + k * RIX + rix), chx32, ldx32, rows32), 0, 0);
 #pragma unroll
-        for (int k = 0; k < CWZ; ++k)
-          zr[k / 2][k % 2] = __builtin_amdgcn_raw_buffer_load_b128(
-              rz, blk_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), J.zoff + 8 * chz, a.ldz, rows, a.sb_shift), 0,
-              0);
-      }
-      return;
+      for (int k = 0; k < CWZ; ++k)
+        zr[k] = __builtin_amdgcn_raw_buffer_load_b128(
+            rz, blk32_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), chz32, ldz32, rows32), 0, 0);
     }
-#pragma unroll
-    for (int i = 0; i < RX; ++i)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t m = (uint32_t)(c * kWC + 2 * (psx + PPX * i) + h);
-        xr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
-            rx, blk_chunk_off(m, J.xoff + 8 * fgx, a.ldx, rows, a.sb_shift), 0, 0);
-      }
-#pragma unroll
-    for (int i = 0; i < RZ; ++i)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t m = (uint32_t)(c * kWC + 2 * (psz + PPZ * i) + h);
-        zr[i][h] = __builtin_amdgcn_raw_buffer_load_b128(
-            rz, blk_chunk_off(m, J.zoff + 8 * fgz, a.ldz, rows, a.sb_shift), 0, 0);
-      }
   };
   auto store_chunk = [&](int buf) {
-    if constexpr (BLK) {
 #pragma unroll
-      for (int k = 0; k < CWX; ++k) {
-        const int col = 2 * (((k * RIX + rix) >> 1) ^ swx);
-        const u32x4_ px = pair_rows<true>(xr[k / 2][k % 2], rix & 1);
+    for (int k = 0; k < CWX; ++k) {
+      const int col = 2 * (((k * RIX + rix) >> 1) ^ swx);
+      const u32x4_ px = pair_rows<true>(xr[k], rix & 1);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t v = px[i];
-          *reinterpret_cast<uint32_t*>(&Xt[buf][8 * chx + 4 * (rix & 1) + i][col]) = v;
-        }
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t v = px[i];
+        *reinterpret_cast<uint32_t*>(&Xt[buf][8 * chx + 4 * (rix & 1) + i][col]) = v;
       }
-#pragma unroll
-      for (int k = 0; k < CWZ; ++k) {
-        const int col = 2 * (((k * RIZ + riz) >> 1) ^ swz);
-        const u32x4_ pz = pair_rows<true>(zr[k / 2][k % 2], riz & 1);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t v = pz[i];
-          *reinterpret_cast<uint32_t*>(&Zt[buf][8 * chz + 4 * (riz & 1) + i][col]) = v;
-        }
-        add_bf16x8(bsum, zr[k / 2][k % 2]);
-      }
-      return;
     }
 #pragma unroll
-    for (int i = 0; i < RX; ++i) {
-      const bf16x8 v0 = __builtin_bit_cast(bf16x8, xr[i][0]), v1 = __builtin_bit_cast(bf16x8, xr[i][1]);
-      const int col = 2 * ((psx + PPX * i) ^ swx);
+    for (int k = 0; k < CWZ; ++k) {
+      const int col = 2 * (((k * RIZ + riz) >> 1) ^ swz);
+      const u32x4_ pz = pair_rows<true>(zr[k], riz & 1);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) *reinterpret_cast<bf16x2*>(&Xt[buf][8 * fgx + e][col]) = bf16x2{v0[e], v1[e]};
-    }
-#pragma unroll
-    for (int i = 0; i < RZ; ++i) {
-      const bf16x8 v0 = __builtin_bit_cast(bf16x8, zr[i][0]), v1 = __builtin_bit_cast(bf16x8, zr[i][1]);
-      const int col = 2 * ((psz + PPZ * i) ^ swz);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        *reinterpret_cast<bf16x2*>(&Zt[buf][8 * fgz + e][col]) = bf16x2{v0[e], v1[e]};
-        bsum[e] += (float)v0[e] + (float)v1[e];
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t v = pz[i];
+        *reinterpret_cast<uint32_t*>(&Zt[buf][8 * chz + 4 * (riz & 1) + i][col]) = v;
       }
+      add_bf16x8(bsum, zr[k]);
     }
   };
   auto row_block = [&](int i) { return kModeN ? i : (wv + 4 * i < KB ? wv + 4 * i : KB - 1); };
@@ -676,7 +625,7 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
       }
     }
   }
-  if (J.bpart && BLK) {
+  if (J.bpart) {
     // the RIZ lanes of a chunk (all in this wave), in a fixed order
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -690,29 +639,6 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
         if (n < J.N) J.bpart[(int64_t)split * J.N + n] = bsum[e];
       }
     }
-  } else if (J.bpart) {
-    // the stores above never touched LDS; the loop ended with a barrier, so Xt is free:
-    // lanes with the same feature group are NFGZ apart (within a wave, then across waves)
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-#pragma unroll
-      for (int msk = NFGZ; msk < 64; msk <<= 1) bsum[e] += shfl_xor(bsum[e], msk);
-    }
-    float* red = reinterpret_cast<float*>(&Xt[0][0][0]);  // [4 waves][NFGZ][8]
-    if (lane < NFGZ) {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) red[(wv * NFGZ + lane) * 8 + e] = bsum[e];
-    }
-    __syncthreads();
-    if (wv == 0 && lane < NFGZ) {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const float v = (red[(0 * NFGZ + lane) * 8 + e] + red[(1 * NFGZ + lane) * 8 + e]) +
-                        (red[(2 * NFGZ + lane) * 8 + e] + red[(3 * NFGZ + lane) * 8 + e]);
-        const int n = 8 * lane + e;
-        if (n < J.N) J.bpart[(int64_t)split * J.N + n] = v;
-      }
-    }
   }
 }
 
@@ -721,10 +647,9 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
 // (128 accumulators), so a k-step reads 6 operands from LDS for 8 MFMAs, and
 // registers remain for two chunks of loads in flight per thread (chunk k is
 // loaded into register set k % 2 two iterations before it goes to LDS).  Same
-// chunking and transposed XOR-swizzled LDS tiles as wgrad_bf16_kernel; its
-// operands are always the blocked buffers of the 8x256 net in bf16 (sample block
-// 32), staged as described at pair_rows: wave w owns feature chunks 4 w .. 4 w +
-// 3, a load covers 16 rows of each.
+// chunking, staging (pair_rows) and transposed XOR-swizzled LDS tiles as
+// wgrad_bf16_kernel: wave w owns feature chunks 4 w .. 4 w + 3, a load covers 16
+// rows of each.
 __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
   constexpr int NFG = 32, CW = NFG / 8, RI = 64 / CW;  // feature chunks, chunks per wave, rows per load
   typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -865,20 +790,15 @@ int wg16_blocks(int n) { return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8; }
 // All jobs of `a` share the tile shape (kb, nb) = wg16_blocks of their 32-feature blocks.
 int launch_wgrad_bf16(const Wgrad16Args& a, int kb, int nb, hipStream_t s) {
   const unsigned grid = (unsigned)(a.splits * a.njobs);
-  const bool blk = a.sb_shift == 5;
-  if (blk != (a.sb_shift != 0)) return -1;  // sample blocks of 1 or 32 only
+  if (a.sb_shift != 5) return -1;  // the kernels stage buffers blocked by 32 samples
   if (kb == 8 && nb == 8) {
-    if (!blk) return -1;  // only the blocked buffers of the 8x256 net have 256 x 256 layers in bf16
     wgrad_bf16_w8_kernel<<<grid, 512, 0, s>>>(a);
     return 0;
   }
-#define NERF_WG16(KB_, NB_)                                                             \
-  if (kb == KB_ && nb == NB_) {                                                         \
-    if (blk)                                                                            \
-      wgrad_bf16_kernel<KB_, NB_, true><<<grid, 256, 0, s>>>(a);                        \
-    else                                                                                \
-      wgrad_bf16_kernel<KB_, NB_, false><<<grid, 256, 0, s>>>(a);                       \
-    return 0;                                                                           \
+#define NERF_WG16(KB_, NB_)                            \
+  if (kb == KB_ && nb == NB_) {                        \
+    wgrad_bf16_kernel<KB_, NB_><<<grid, 256, 0, s>>>(a); \
+    return 0;                                          \
   }
 #define NERF_WG16_ROW(KB_) NERF_WG16(KB_, 1) NERF_WG16(KB_, 2) NERF_WG16(KB_, 4) NERF_WG16(KB_, 8)
   NERF_WG16_ROW(1) NERF_WG16_ROW(2) NERF_WG16_ROW(4)
@@ -1065,8 +985,8 @@ int launch_bwd_bf16_any(const MlpPlan& P, const void* packed, int64_t M, const f
     return P.sblk != 32 ? -1
                         : launch_bwd_bf16ws8<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
   else
-    return P.sblk != 1 ? -1
-                       : launch_bwd_bf16<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
+    return P.sblk != 32 ? -1
+                        : launch_bwd_bf16<W, D, SKIP, LX, LD>(P, packed, M, d_rgb, d_sigma, rgb, sigma, stash, dz, s);
 }
 
 int dispatch_bwd(const MlpPlan& P, const void* packed, int64_t M, const float* d_rgb,

@@ -184,7 +184,7 @@ int nerf_dispatch_fwd(const MlpPlan& P, const void* packed, int64_t M, const flo
                                                sigma, stash, s);                              \
       return 0;                                                                               \
     }                                                                                         \
-    if (P.sblk != 1) return -1;                                                               \
+    if (P.sblk != 32) return -1;                                                              \
     return launch_fwd_bf16<W_, D_, S_, LX_, LD_>(P, packed, M, xyz, dirs, ro, rd, tv, spr,    \
                                                  rgb, sigma, stash, s);                       \
   }

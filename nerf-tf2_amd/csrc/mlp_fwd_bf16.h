@@ -22,6 +22,10 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16_kernel(
   const int64_t m = blk * 32 + j;
   const bool valid = m < M;
   const int64_t mm = valid ? m : M - 1;
+  // this sample's row of the stash, blocked by 32 samples (the wave's block)
+  const BlkRow<__bf16> srow{stash ? stash + blk * 32 * P.stash_per_sample : nullptr,
+                            (int)(M - blk * 32 < 32 ? M - blk * 32 : 32), j};
+  const bool st = stash && valid;
   float x[3], d[3];
   load_sample(mm, xyz, dirs, ro, rd, tv, spr, x, d);
   bf16x8 exb[C::KX / 8], edb[C::KDA / 8];
@@ -29,15 +33,14 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16_kernel(
     float ex[C::KX];
     encode_slots<LX, C::KX>(x, g, ex);
     slots_to_bf16(ex, exb);
-    if (stash && valid) store_slots_bf16<C::KX>(ex, stash + mm * P.stash_per_sample + P.st_encx + g * C::KX);
+    if (st) store_slots_bf16<C::KX>(ex, srow, P.st_encx + g * C::KX);
   }
   if constexpr (C::RGB) {
     float ed[C::KDA];
     encode_slots<LD, C::KDA>(d, g, ed);
     slots_to_bf16(ed, edb);
-    if (stash && valid) store_slots_bf16<C::KDA>(ed, stash + mm * P.stash_per_sample + P.st_encd + g * C::KDA);
+    if (st) store_slots_bf16<C::KDA>(ed, srow, P.st_encd + g * C::KDA);
   }
-  __bf16* srow = (stash && valid) ? stash + mm * P.stash_per_sample : nullptr;
 
   floatx16 acc[C::NB];
   zero_acc(acc);
@@ -45,7 +48,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16_kernel(
   bias_act<C::NB, 1>(acc, reinterpret_cast<const float*>(pk + P.p_bias[0]), g);
   wave_sync();
   store_act_bf16(acc, act, lane);
-  if (srow) store_rows_bf16(acc, srow + P.st_h[0], g);
+  if (st) store_rows_bf16(acc, srow, P.st_h[0], g);
 #pragma unroll
   for (int l = 1; l < D; ++l) {
     const bf16x8* wq = reinterpret_cast<const bf16x8*>(pk + P.p_fwd[l]);
@@ -56,7 +59,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16_kernel(
     bias_act<C::NB, 1>(acc, reinterpret_cast<const float*>(pk + P.p_bias[l]), g);
     wave_sync();
     store_act_bf16(acc, act, lane);
-    if (srow) store_rows_bf16(acc, srow + P.st_h[l], g);
+    if (st) store_rows_bf16(acc, srow, P.st_h[l], g);
   }
   // heads see the bf16-rounded activations, like the following layers
   round_acc_bf16(acc);
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16_kernel(
     bias_act<C::NB, 0>(acc, reinterpret_cast<const float*>(pk + P.p_bias[mb]), g);
     wave_sync();
     store_act_bf16(acc, act, lane);
-    if (srow) store_rows_bf16(acc, srow + P.st_bott, g);
+    if (st) store_rows_bf16(acc, srow, P.st_bott, g);
     const int mr = D + 1;
     const bf16x8* wq = reinterpret_cast<const bf16x8*>(pk + P.p_fwd[mr]);
     floatx16 accr[C::NBR];
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bf16_kernel(
     mfma_regs_bf16<C::NBR, C::KDA / 8>(accr, wq + TH * C::NBR * 64, edb, lane);
     bias_act<C::NBR, 1>(accr, reinterpret_cast<const float*>(pk + P.p_bias[mr]), g);
     round_acc_bf16(accr);
-    if (srow) store_rows_bf16(accr, srow + P.st_rbr, g);
+    if (st) store_rows_bf16(accr, srow, P.st_rbr, g);
     rgb_head_f32<C::NBR>(accr, reinterpret_cast<const float*>(pk + P.p_head_rgb), g, rgb);
   }
   if (valid && g == 0) {

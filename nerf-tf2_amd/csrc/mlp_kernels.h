@@ -406,26 +406,40 @@ __device__ __forceinline__ void store_act_bf16(const floatx16 (&acc)[NBo], bf16x
   }
 }
 
-// Accumulators -> a row-major bf16 feature row (natural order), 4 features per store.
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4_ __attribute__((ext_vector_type(4)));
+
+// One sample's row of a bf16 stash or dz buffer blocked by 32 samples
+// (MlpPlan::sblk): the sample is row j of the block of n rows starting at blk,
+// whose 8-element chunk c of that row is at blk + 8 (c n + j).
+template <typename T>
+struct BlkRow {
+  T* blk;
+  int n, j;
+  __device__ __forceinline__ T* at(int64_t e) const { return blk + 8 * ((e >> 3) * n + j) + (e & 7); }
+};
+
+// Accumulators -> features off .. of a blocked bf16 row (natural order), 4
+// features per store; lane halves g = 0, 1 fill the two halves of each 8-feature
+// chunk, so a store of the wave's 32 samples covers 512 contiguous bytes.
 template <int NBo>
-__device__ __forceinline__ void store_rows_bf16(const floatx16 (&acc)[NBo], __bf16* row, int g) {
+__device__ __forceinline__ void store_rows_bf16(const floatx16 (&acc)[NBo], BlkRow<__bf16> row, int64_t off, int g) {
 #pragma unroll
   for (int nb = 0; nb < NBo; ++nb)
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const floatx4_ v = {acc[nb][4 * a], acc[nb][4 * a + 1], acc[nb][4 * a + 2], acc[nb][4 * a + 3]};
-      *reinterpret_cast<bf16x4*>(row + 32 * nb + 8 * a + 4 * g) = __builtin_convertvector(v, bf16x4);
+      *reinterpret_cast<bf16x4*>(row.at(off + 32 * nb + 8 * a + 4 * g)) = __builtin_convertvector(v, bf16x4);
     }
 }
 
+// Encoding slots v (K a multiple of 4) -> elements off .. off + K - 1 of a blocked bf16 row.
 template <int K>
-__device__ __forceinline__ void store_slots_bf16(const float (&v)[K], __bf16* dst) {
+__device__ __forceinline__ void store_slots_bf16(const float (&v)[K], BlkRow<__bf16> row, int64_t off) {
 #pragma unroll
   for (int e = 0; e < K; e += 4) {
     const floatx4_ x = {v[e], v[e + 1], v[e + 2], v[e + 3]};
-    *reinterpret_cast<bf16x4*>(dst + e) = __builtin_convertvector(x, bf16x4);
+    *reinterpret_cast<bf16x4*>(row.at(off + e)) = __builtin_convertvector(x, bf16x4);
   }
 }
 

@@ -129,9 +129,8 @@ struct MlpPlan {
   // elements per sample is cut into blocks of sblk consecutive samples (the last
   // one may hold fewer); a block [b, b + n) stores its rows chunk-major: the
   // 8-element chunk c of sample m at element b S + 8 (c n + m - b).  sblk = 1 is
-  // the plain row-major layout; 32 (one MFMA sample block) for the bf16
-  // weight-stationary kernels of the 8x256 NeRF MLP, whose 16-byte row chunks
-  // of 32 consecutive samples are then 512 contiguous bytes.
+  // the plain row-major layout (fp32); 32 (one MFMA sample block) for bf16, whose
+  // row chunks of 32 consecutive samples are then 512 contiguous bytes.
   int sblk;
 };
 
@@ -152,8 +151,8 @@ inline int mlp_make_plan(int D, int W, int skip, int LX, int LD, int prec, MlpPl
   MlpPlan& P = *p;
   P = MlpPlan{};
   P.D = D; P.W = W; P.skip = skip; P.LX = LX; P.LD = LD; P.rgb = LD > 0; P.prec = prec;
-  // the bf16 weight-stationary kernels' topology (their launchers check it)
-  P.sblk = (prec == 1 && W == 256 && D == 8 && LX == 10 && LD == 4 && skip >= 0 && skip + 1 < D) ? 32 : 1;
+  // every bf16 stash and dz is blocked (the kernels' launchers check it)
+  P.sblk = prec == 1 ? 32 : 1;
   P.NB = W / 32;
   P.WR = W / 2;
   P.NBR = (P.WR + 31) / 32;

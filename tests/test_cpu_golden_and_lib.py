@@ -60,10 +60,11 @@ def test_size_queries(nerf):
     # dz and stash rows start every segment on 8 elements (16-byte bf16 row chunks)
     assert lib.nerf_mlp_dz_floats_per_sample(ctypes.byref(d)) % 8 == 0
     assert lib.nerf_mlp_stash_floats_per_sample(ctypes.byref(d)) % 8 == 0
-    # stash / dz sample blocks: 32 for the bf16 weight-stationary kernels of the reference net, else plain rows
+    # stash / dz sample blocks: 32 for bf16, plain rows for fp32
     assert lib.nerf_mlp_sample_block(ctypes.byref(d), 1) == 32
     assert lib.nerf_mlp_sample_block(ctypes.byref(d), 0) == 1
-    assert lib.nerf_mlp_sample_block(ctypes.byref(prop), 1) == 1
+    assert lib.nerf_mlp_sample_block(ctypes.byref(prop), 1) == 32
+    assert lib.nerf_mlp_sample_block(ctypes.byref(prop), 0) == 1
     assert lib.nerf_mlp_sample_block(ctypes.byref(bad), 1) == -1
 
 

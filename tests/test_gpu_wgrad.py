@@ -4,9 +4,9 @@ stash and dz of one BF16 forward + dX chain, widened exactly to float32, give th
 FP32 kernels the very products the BF16 kernels form (bf16 x bf16 is exact in
 float32), so the two gradients differ only in the float32 summation order.  Both
 precisions share one stash / dz column layout; the FP32 kernels read only the
-columns of each layer's operands.  The BF16 buffers of the reference net are
-blocked by 32 samples (nerf_mlp_sample_block); they are turned back into plain
-rows for the FP32 kernels, which also pins that layout."""
+columns of each layer's operands.  The BF16 buffers are blocked by 32 samples
+(nerf_mlp_sample_block); they are turned back into plain rows for the FP32
+kernels, which also pins that layout."""
 import ctypes
 
 import pytest
@@ -66,7 +66,7 @@ def test_bf16_weight_grads_match_fp32_kernels(nerf, net, M):
 
     gb = weight_grads(bf, stash, dz)
     B = bf.sample_block
-    assert B == (32 if net == "nerf" else 1) and f32.sample_block == 1
+    assert B == 32 and f32.sample_block == 1
     # the ReLU bit words in the bf16 stash are not numbers; no kernel reads them as operands
     gf = weight_grads(f32, torch.nan_to_num(unblock(stash, B).float(), nan=0.0, posinf=0.0, neginf=0.0).contiguous(),
                       unblock(dz, B).float().contiguous())
