@@ -175,8 +175,8 @@ int64_t nerf_mlp_dz_floats_per_sample(const nerf_mlp_desc* desc);
 /* Samples per block of the stash and dz buffers for `precision` (-1 for an
  * unsupported description): both hold M * (elements per sample) elements, cut
  * into blocks of this many consecutive samples, each block stored chunk-major
- * (1: plain rows; 32 for the 8x256 reference MLP in bf16, so that the kernels
- * write and read whole 512-byte stretches).  A batch whose weight gradients are
+ * (1: plain rows, for float32; 32 for bf16, so that the kernels write and
+ * read whole 512-byte stretches).  A batch whose weight gradients are
  * summed over several nerf_mlp_weight_grads calls (accumulate) is split at
  * multiples of it.  No counterpart in the reference (TF keeps activations in
  * its own tensors). */
