@@ -12,6 +12,7 @@
 //   Every pre-activation gradient dz_l is written row-major for the weight GEMMs.
 // nerf_mlp_weight_grads: dW_l = X_l^T dz_l and db_l = sum dz_l over the samples,
 //   as a split-K MFMA GEMM whose partial tiles are summed in a fixed order.
+#include <atomic>
 #include <mutex>
 
 #include "common.h"
@@ -495,16 +496,34 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const __bf16* base, 
 // chunks), and the transposed LDS tile stores sample pair p of feature row R at
 // pair position p ^ 4 ((R >> 4) & 7), which keeps every 8-sample MFMA operand
 // contiguous and spreads the pair writes of a wave over the banks.
+// The LDS tiles hold 8 NFG feature rows of each operand (NFG = the operand's
+// 8-feature chunks), double-buffered except for the 64 x 256 and 256 x 64
+// tiles: their registers allow two workgroups per CU, which two buffers (92 KB)
+// would not; these jobs stream their rows at one chunk of loads in flight per
+// thread, and the second workgroup doubles the bytes in flight (the 64 x 256
+// encoding-slot jobs of the 8x256 net: 220 -> 150 us for 720,896 samples) for
+// one more barrier per chunk.
+__host__ __device__ constexpr int wg16_nfg(int b) { return b >= 8 ? 32 : b >= 4 ? 16 : 8; }
+__host__ __device__ constexpr int wg16_nbuf(int kb, int nb) { return kb * nb == 16 && kb != 4 ? 1 : 2; }
+// Register sets of loads in flight: two where one set is at most 10 16-byte
+// loads and the tile at most 8 blocks (the registers then stay within 256);
+// the larger tiles would spill with two.
+__host__ __device__ constexpr int wg16_nset(int kb, int nb) {
+  return kb * nb <= 8 && wg16_nfg(kb) / 4 + wg16_nfg(nb) / 4 <= 10 ? 2 : 1;
+}
+
 template <int KB, int NB>
 __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
   constexpr bool kModeN = NB >= 4;
   constexpr int RB = kModeN ? KB : (KB >= 8 ? 2 : 1);  // row blocks per wave
   constexpr int CB = kModeN ? NB / 4 : NB;             // column blocks per wave
-  constexpr int NFGX = KB >= 8 ? 32 : KB >= 4 ? 16 : 8, NFGZ = NB >= 8 ? 32 : NB >= 4 ? 16 : 8;
+  constexpr int NFGX = wg16_nfg(KB), NFGZ = wg16_nfg(NB);
   constexpr int CWX = NFGX / 4, CWZ = NFGZ / 4;  // chunks per wave
   constexpr int RIX = 64 / CWX, RIZ = 64 / CWZ;  // rows per load
-  __shared__ __attribute__((aligned(16))) __bf16 Xt[2][kWF][kWP];
-  __shared__ __attribute__((aligned(16))) __bf16 Zt[2][kWF][kWP];
+  constexpr int NBUF = wg16_nbuf(KB, NB);
+  constexpr int NSET = wg16_nset(KB, NB);
+  __shared__ __attribute__((aligned(16))) __bf16 Xt[NBUF][8 * NFGX][kWP];
+  __shared__ __attribute__((aligned(16))) __bf16 Zt[NBUF][8 * NFGZ][kWP];
   const int split = blockIdx.x % a.splits;
   const Wgrad16Job J = a.job[blockIdx.x / a.splits];
   const int64_t mb = (int64_t)split * a.m_per_split;
@@ -530,35 +549,37 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) bsum[e] = 0.f;
 
-  u32x4_ xr[CWX], zr[CWZ];
+  u32x4_ xr[NSET][CWX], zr[NSET][CWZ];  // NSET chunks of loads in flight
   const uint32_t bxf = (uint32_t)((((J.xoff >> 3) + chx) * 32 + rix) * 16);  // fast-path lane bases
   const uint32_t bzf = (uint32_t)((((J.zoff >> 3) + chz) * 32 + riz) * 16);
   const uint32_t chx32 = (uint32_t)((J.xoff >> 3) + chx), chz32 = (uint32_t)((J.zoff >> 3) + chz);
   const uint32_t ldx32 = (uint32_t)a.ldx, ldz32 = (uint32_t)a.ldz, rows32 = (uint32_t)rows;
-  auto load_chunk = [&](int c) {
+  auto load_chunk = [&](auto set, int c) {
+    constexpr int S = decltype(set)::value;
     if ((int64_t)(c + 1) * kWC <= (rows & ~(int64_t)31)) {  // whole blocks (wave-uniform)
 #pragma unroll
       for (int k = 0; k < CWX; ++k)
-        xr[k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, 0);
+        xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, 0);
 #pragma unroll
       for (int k = 0; k < CWZ; ++k)
-        zr[k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, 0);
+        zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, 0);
     } else {
 #pragma unroll
       for (int k = 0; k < CWX; ++k)
-        xr[k] = __builtin_amdgcn_raw_buffer_load_b128(
+        xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(
             rx, blk32_chunk_off((uint32_t)(c * kWC + k * RIX + rix), chx32, ldx32, rows32), 0, 0);
 #pragma unroll
       for (int k = 0; k < CWZ; ++k)
-        zr[k] = __builtin_amdgcn_raw_buffer_load_b128(
+        zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(
             rz, blk32_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), chz32, ldz32, rows32), 0, 0);
     }
   };
-  auto store_chunk = [&](int buf) {
+  auto store_chunk = [&](auto set, int buf) {
+    constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int k = 0; k < CWX; ++k) {
       const int col = 2 * (((k * RIX + rix) >> 1) ^ swx);
-      const u32x4_ px = pair_rows<true>(xr[k], rix & 1);
+      const u32x4_ px = pair_rows<true>(xr[S][k], rix & 1);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const uint32_t v = px[i];
@@ -568,13 +589,13 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
 #pragma unroll
     for (int k = 0; k < CWZ; ++k) {
       const int col = 2 * (((k * RIZ + riz) >> 1) ^ swz);
-      const u32x4_ pz = pair_rows<true>(zr[k], riz & 1);
+      const u32x4_ pz = pair_rows<true>(zr[S][k], riz & 1);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const uint32_t v = pz[i];
         *reinterpret_cast<uint32_t*>(&Zt[buf][8 * chz + 4 * (riz & 1) + i][col]) = v;
       }
-      add_bf16x8(bsum, zr[k]);
+      add_bf16x8(bsum, zr[S][k]);
     }
   };
   auto row_block = [&](int i) { return kModeN ? i : (wv + 4 * i < KB ? wv + 4 * i : KB - 1); };
@@ -598,16 +619,48 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
     }
   };
 
+  // chunk k goes to LDS buffer k % 2 (or, after a barrier, the one buffer) from
+  // register set k % NSET, loaded NSET chunks before
   const int nchunks = (int)((rows + kWC - 1) / kWC);
-  load_chunk(0);
-  store_chunk(0);
-  load_chunk(1);
-  __syncthreads();
-  for (int c = 0; c < nchunks; ++c) {
-    compute(c & 1);
-    store_chunk((c + 1) & 1);  // chunk c+1 (zeros past the end; never read then)
-    load_chunk(c + 2);
+  const std::integral_constant<int, 0> set0;
+  if constexpr (NSET == 2) {
+    const std::integral_constant<int, NSET - 1> set1;
+    constexpr int B1 = NBUF == 2 ? 1 : 0;
+    load_chunk(set0, 0);
+    load_chunk(set1, 1);
+    store_chunk(set0, 0);
+    load_chunk(set0, 2);
     __syncthreads();
+    for (int c = 0; c < nchunks; c += 2) {
+      compute(0);  // chunk c
+      if constexpr (NBUF == 1) __syncthreads();
+      store_chunk(set1, B1);  // chunk c+1 (zeros past the end; never read then)
+      load_chunk(set1, c + 3);
+      __syncthreads();
+      if (c + 1 >= nchunks) break;
+      compute(B1);  // chunk c+1
+      if constexpr (NBUF == 1) __syncthreads();
+      store_chunk(set0, 0);  // chunk c+2
+      load_chunk(set0, c + 4);
+      __syncthreads();
+    }
+  } else {
+    load_chunk(set0, 0);
+    store_chunk(set0, 0);
+    load_chunk(set0, 1);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+      if constexpr (NBUF == 2) {
+        compute(c & 1);
+        store_chunk(set0, (c + 1) & 1);  // chunk c+1 (zeros past the end; never read then)
+      } else {
+        compute(0);
+        __syncthreads();
+        store_chunk(set0, 0);
+      }
+      load_chunk(set0, c + 2);
+      __syncthreads();
+    }
   }
 
   float* out = J.part + (int64_t)split * J.K * J.N;
@@ -786,6 +839,29 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
 }
 
 int wg16_blocks(int n) { return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8; }
+
+// Workgroups of the generic kernel for tile shape (kb, nb) that a CU holds at
+// once (registers and LDS, from the runtime's occupancy query; 1 if it fails),
+// at most 3: the splits of a narrow group scale with it.
+int wg16_per_cu(int kb, int nb) {
+  static std::atomic<int> memo[4][4];  // 0: not asked yet
+  const int i = kb >= 8 ? 3 : kb >= 4 ? 2 : kb - 1, j = nb >= 8 ? 3 : nb >= 4 ? 2 : nb - 1;
+  int v = memo[i][j].load(std::memory_order_relaxed);
+  if (v) return v;
+  const void* fn = nullptr;
+#define NERF_WG16_FN(KB_, NB_) \
+  if (kb == KB_ && nb == NB_) fn = reinterpret_cast<const void*>(&wgrad_bf16_kernel<KB_, NB_>);
+#define NERF_WG16_FN_ROW(KB_) NERF_WG16_FN(KB_, 1) NERF_WG16_FN(KB_, 2) NERF_WG16_FN(KB_, 4) NERF_WG16_FN(KB_, 8)
+  NERF_WG16_FN_ROW(1) NERF_WG16_FN_ROW(2) NERF_WG16_FN_ROW(4)
+  NERF_WG16_FN(8, 1) NERF_WG16_FN(8, 2) NERF_WG16_FN(8, 4)
+#undef NERF_WG16_FN_ROW
+#undef NERF_WG16_FN
+  int n = 1;
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
+  v = n < 3 ? n : 3;
+  memo[i][j].store(v, std::memory_order_relaxed);
+  return v;
+}
 
 // All jobs of `a` share the tile shape (kb, nb) = wg16_blocks of their 32-feature blocks.
 int launch_wgrad_bf16(const Wgrad16Args& a, int kb, int nb, hipStream_t s) {
@@ -1052,6 +1128,15 @@ struct Wg16Group {
   int64_t sg;
 };
 
+static int64_t wg16_group_floats(const Wg16Group& g, const WgradJob* jobs) {
+  int64_t f = 0;
+  for (int i = 0; i < g.n; ++i) {
+    const WgradJob& J = jobs[g.members[i]];
+    f += g.sg * J.K * J.N + (J.bias ? g.sg * J.N : 0);
+  }
+  return f;
+}
+
 static int wg16_groups(const MlpPlan& P, int64_t M, const WgradJob* jobs, int nj, Wg16Group* groups) {
   bool done[2 * NERF_MAX_LAYERS] = {};
   const int64_t row_bytes = 2 * (P.stash_per_sample > P.dz_per_sample ? P.stash_per_sample : P.dz_per_sample);
@@ -1071,16 +1156,30 @@ static int wg16_groups(const MlpPlan& P, int64_t M, const WgradJob* jobs, int nj
     g.sg = wg16_splits(M, g.n);
     if (g.sg < wg16_min_splits(M, row_bytes)) g.sg = wg16_min_splits(M, row_bytes);
   }
-  return ng;
-}
-
-static int64_t wg16_group_floats(const Wg16Group& g, const WgradJob* jobs) {
-  int64_t f = 0;
-  for (int i = 0; i < g.n; ++i) {
-    const WgradJob& J = jobs[g.members[i]];
-    f += g.sg * J.K * J.N + (J.bias ? g.sg * J.N : 0);
+  // Where the generic kernel fits several workgroups per CU, a group gets that
+  // many times the splits, as far as its partial tiles stay within the largest
+  // group's (the workspace a call uses does not grow) and its tiles are large
+  // enough (16,384 floats per split) for the split sums to run wide: the sums
+  // over the splits of a narrow tile (one thread per output) cost more than
+  // the extra workgroups gain.
+  int64_t most = 0;
+  for (int i = 0; i < ng; ++i) {
+    const int64_t f = wg16_group_floats(groups[i], jobs);
+    if (f > most) most = f;
   }
-  return f;
+  const int64_t cap = (M + 255) / 256;
+  for (int i = 0; i < ng; ++i) {
+    Wg16Group& g = groups[i];
+    if (g.kb == 8 && g.nb == 8) continue;
+    int64_t per = 0;  // floats per split
+    for (int k = 0; k < g.n; ++k) per += jobs[g.members[k]].K * (int64_t)jobs[g.members[k]].N + jobs[g.members[k]].N;
+    if (per < 16384) continue;
+    int64_t sg = (256 * wg16_per_cu(g.kb, g.nb) + g.n - 1) / g.n;
+    if (sg > cap) sg = cap;
+    if (sg > most / per) sg = most / per;
+    if (sg > g.sg) g.sg = sg;
+  }
+  return ng;
 }
 
 // Workspace floats the launches of one call use (M > 0), or -1 for a layer the
