@@ -34,6 +34,14 @@ void nerf_set_error(const char* fmt, ...);
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Cache policy of buffer loads and stores that stream a buffer no kernel reads
+// again soon (the blocked BF16 stash and dz, written once and read once a kernel
+// or two later, far beyond the L2): non-temporal (`nt` on gfx950), so that they
+// do not displace the weights and tiles the kernels do reuse.  Only for stores of
+// whole cache lines: the FP32 stash and dz rows, stored in 64-byte pieces that
+// the L2 merges, take longer with it.
+constexpr int kNT = 2;
+
 // ---------------------------------------------------------------------------
 // Exactly rounded float ops that the compiler may not contract or reorder.
 // ---------------------------------------------------------------------------

@@ -559,19 +559,19 @@ __global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
     if ((int64_t)(c + 1) * kWC <= (rows & ~(int64_t)31)) {  // whole blocks (wave-uniform)
 #pragma unroll
       for (int k = 0; k < CWX; ++k)
-        xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, 0);
+        xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, kNT);
 #pragma unroll
       for (int k = 0; k < CWZ; ++k)
-        zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, 0);
+        zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, kNT);
     } else {
 #pragma unroll
       for (int k = 0; k < CWX; ++k)
         xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(
-            rx, blk32_chunk_off((uint32_t)(c * kWC + k * RIX + rix), chx32, ldx32, rows32), 0, 0);
+            rx, blk32_chunk_off((uint32_t)(c * kWC + k * RIX + rix), chx32, ldx32, rows32), 0, kNT);
 #pragma unroll
       for (int k = 0; k < CWZ; ++k)
         zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(
-            rz, blk32_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), chz32, ldz32, rows32), 0, 0);
+            rz, blk32_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), chz32, ldz32, rows32), 0, kNT);
     }
   };
   auto store_chunk = [&](auto set, int buf) {
@@ -740,8 +740,8 @@ __global__ __launch_bounds__(512) void wgrad_bf16_w8_kernel(Wgrad16Args a) {
 #pragma unroll
     for (int k = 0; k < CW; ++k) {
       const uint32_t row = (uint32_t)(c * kWC + k * RI + rin);
-      xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk32_chunk_off(row, chx32, ldx32, rows32), 0, 0);
-      zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk32_chunk_off(row, chz32, ldz32, rows32), 0, 0);
+      xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk32_chunk_off(row, chx32, ldx32, rows32), 0, kNT);
+      zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk32_chunk_off(row, chz32, ldz32, rows32), 0, kNT);
     }
   };
   auto store_chunk = [&](auto set, int buf) {
@@ -905,10 +905,18 @@ __global__ void zero_kernel(float* __restrict__ p, int64_t n) {
 }
 
 // Sum of n partials p[0], p[stride], ... in sequential order; the loads are
-// issued 16 at a time so that they are in flight together.
+// issued 64 (then 16) at a time so that they are in flight together: a tile with
+// few outputs has one thread per output, whose loads are the whole launch's.
 __device__ __forceinline__ float sum_splits(const float* __restrict__ p, int64_t stride, int n) {
   float s = 0.f;
   int sp = 0;
+  for (; sp + 64 <= n; sp += 64) {
+    float v[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) v[j] = p[(int64_t)(sp + j) * stride];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) s += v[j];
+  }
   for (; sp + 16 <= n; sp += 16) {
     float v[16];
 #pragma unroll

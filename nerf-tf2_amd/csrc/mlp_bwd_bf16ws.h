@@ -15,7 +15,7 @@ using namespace ws;
 // store and a VALU write of its data registers, and on gfx950 such stores have
 // written another value for some lanes.
 __device__ __forceinline__ void st4(bf16x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff + (uint32_t)soff, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff + (uint32_t)soff, 0, kNT);
 }
 
 // v with the ReLU mask applied: the forward's bit mask w holds, for the pairs of
@@ -34,7 +34,7 @@ __device__ __forceinline__ uint32_t ld_u32(__amdgpu_buffer_rsrc_t r, uint32_t vo
 
 // Stores a 16-feature group given in the accumulator layout as row chunks.
 __device__ __forceinline__ void st_group(bf16x8 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int64_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, swap_halves(v)), r, voff + (uint32_t)soff, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, swap_halves(v)), r, voff + (uint32_t)soff, 0, kNT);
 }
 
 }  // namespace wsb

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
   };
   auto uoff = [&](int sb, int64_t u) { return (int)(16 * rows_in(sb) * (int)(u >> 3) + 2 * (int)(u & 7)); };
   auto st16 = [&](bf16x8 v, uint32_t vo, int so) {
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(ws::u32x4, v), rst, vo + (uint32_t)so, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(ws::u32x4, v), rst, vo + (uint32_t)so, 0, kNT);
   };
   auto load_half = [&](bf16x8 (&w)[kHalf], int m, bool enc, bool second) {
     if (enc) {
@@ -175,11 +175,11 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
 #pragma unroll
       for (int e = 0; e < C::KX; e += 4)
         __builtin_amdgcn_raw_buffer_store_b64(ws::bf16x4_bits(ex[e], ex[e + 1], ex[e + 2], ex[e + 3]), rst,
-                                              vx + (uint32_t)uoff(wv, P.st_encx + e), 0, 0);
+                                              vx + (uint32_t)uoff(wv, P.st_encx + e), 0, kNT);
 #pragma unroll
       for (int e = 0; e < C::KD; e += 4)
         __builtin_amdgcn_raw_buffer_store_b64(ws::bf16x4_bits(ed[e], ed[e + 1], ed[e + 2], ed[e + 3]), rst,
-                                              vd + (uint32_t)uoff(wv, P.st_encd + e), 0, 0);
+                                              vd + (uint32_t)uoff(wv, P.st_encd + e), 0, kNT);
     }
   }
 
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
       const ws::u32x2 v = {mw[sb][0], mw[sb][1]};
       // (2 nb0 + g) D = 16 nb0 + 8 g: lane half g one chunk on
       __builtin_amdgcn_raw_buffer_store_b64(
-          v, rst, voff(sb, g, 0) + (uint32_t)uoff(sb, P.st_mask + 2 * nb0 * D + 4 * q), 0, 0);
+          v, rst, voff(sb, g, 0) + (uint32_t)uoff(sb, P.st_mask + 2 * nb0 * D + 4 * q), 0, kNT);
     }
   };
 
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(512) void mlp_fwd_bf16ws8_kernel(
         // ReLU bits of rgb-branch block wv: bits 0..7 and 16..23 of nz_bits as 16 bits
         const uint32_t b = ws::nz_bits(lo, hi);
         __builtin_amdgcn_raw_buffer_store_b16((unsigned short)((b & 0xFFu) | ((b >> 8) & 0xFF00u)), rst,
-                                              voff(sb, 0, g) + (uint32_t)uoff(sb, P.st_mask + 16 * D + 2 * wv), 0, 0);
+                                              voff(sb, 0, g) + (uint32_t)uoff(sb, P.st_mask + 16 * D + 2 * wv), 0, kNT);
       }
       float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll
