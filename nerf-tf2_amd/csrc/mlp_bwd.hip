@@ -497,14 +497,20 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const __bf16* base, 
 // pair position p ^ 4 ((R >> 4) & 7), which keeps every 8-sample MFMA operand
 // contiguous and spreads the pair writes of a wave over the banks.
 // The LDS tiles hold 8 NFG feature rows of each operand (NFG = the operand's
-// 8-feature chunks), double-buffered except for the 64 x 256 and 256 x 64
-// tiles: their registers allow two workgroups per CU, which two buffers (92 KB)
-// would not; these jobs stream their rows at one chunk of loads in flight per
-// thread, and the second workgroup doubles the bytes in flight (the 64 x 256
-// encoding-slot jobs of the 8x256 net: 220 -> 150 us for 720,896 samples) for
-// one more barrier per chunk.
+// 8-feature chunks), double-buffered except for the 64 x 256, 256 x 64 and
+// 256 x 128 tiles: their registers allow two workgroups per CU, which two
+// buffers (92-110 KB) would not; these jobs stream their rows at one chunk of
+// loads in flight per thread, and the second workgroup doubles the bytes in
+// flight (the 64 x 256 encoding-slot jobs of the 8x256 net: 220 -> 150 us for
+// 720,896 samples) for one more barrier per chunk.
 __host__ __device__ constexpr int wg16_nfg(int b) { return b >= 8 ? 32 : b >= 4 ? 16 : 8; }
-__host__ __device__ constexpr int wg16_nbuf(int kb, int nb) { return kb * nb == 16 && kb != 4 ? 1 : 2; }
+__host__ __device__ constexpr int wg16_nbuf(int kb, int nb) {
+  return (kb * nb == 16 && kb != 4) || (kb == 8 && nb == 4) ? 1 : 2;
+}
+// Waves per SIMD the register allocation must allow: two for the 256 x 128 tile
+// (128 accumulators a wave; 247 registers then, 288 otherwise, which ran one
+// workgroup per CU).  The 128 x 256 tile would spill under the same bound.
+__host__ __device__ constexpr int wg16_min_waves(int kb, int nb) { return kb == 8 && nb == 4 ? 2 : 1; }
 // Register sets of loads in flight: two where one set is at most 10 16-byte
 // loads and the tile at most 8 blocks (the registers then stay within 256);
 // the larger tiles would spill with two.
@@ -513,7 +519,7 @@ __host__ __device__ constexpr int wg16_nset(int kb, int nb) {
 }
 
 template <int KB, int NB>
-__global__ __launch_bounds__(256) void wgrad_bf16_kernel(Wgrad16Args a) {
+__global__ __launch_bounds__(256, wg16_min_waves(KB, NB)) void wgrad_bf16_kernel(Wgrad16Args a) {
   constexpr bool kModeN = NB >= 4;
   constexpr int RB = kModeN ? KB : (KB >= 8 ? 2 : 1);  // row blocks per wave
   constexpr int CB = kModeN ? NB / 4 : NB;             // column blocks per wave

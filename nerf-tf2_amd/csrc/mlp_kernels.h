@@ -421,7 +421,9 @@ struct BlkRow {
 
 // Accumulators -> features off .. of a blocked bf16 row (natural order), 4
 // features per store; lane halves g = 0, 1 fill the two halves of each 8-feature
-// chunk, so a store of the wave's 32 samples covers 512 contiguous bytes.
+// chunk, so a store of the wave's 32 samples covers 512 contiguous bytes (whole
+// cache lines: stored non-temporally, as the stash and dz are read back only
+// after the L2 has turned over).
 template <int NBo>
 __device__ __forceinline__ void store_rows_bf16(const floatx16 (&acc)[NBo], BlkRow<__bf16> row, int64_t off, int g) {
 #pragma unroll
@@ -429,7 +431,8 @@ __device__ __forceinline__ void store_rows_bf16(const floatx16 (&acc)[NBo], BlkR
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const floatx4_ v = {acc[nb][4 * a], acc[nb][4 * a + 1], acc[nb][4 * a + 2], acc[nb][4 * a + 3]};
-      *reinterpret_cast<bf16x4*>(row.at(off + 32 * nb + 8 * a + 4 * g)) = __builtin_convertvector(v, bf16x4);
+      __builtin_nontemporal_store(__builtin_convertvector(v, bf16x4),
+                                  reinterpret_cast<bf16x4*>(row.at(off + 32 * nb + 8 * a + 4 * g)));
     }
 }
 
@@ -439,7 +442,7 @@ __device__ __forceinline__ void store_slots_bf16(const float (&v)[K], BlkRow<__b
 #pragma unroll
   for (int e = 0; e < K; e += 4) {
     const floatx4_ x = {v[e], v[e + 1], v[e + 2], v[e + 3]};
-    *reinterpret_cast<bf16x4*>(row.at(off + e)) = __builtin_convertvector(x, bf16x4);
+    __builtin_nontemporal_store(__builtin_convertvector(x, bf16x4), reinterpret_cast<bf16x4*>(row.at(off + e)));
   }
 }
 
