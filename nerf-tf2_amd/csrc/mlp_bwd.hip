@@ -863,7 +863,11 @@ int wg16_per_cu(int kb, int nb) {
 #undef NERF_WG16_FN_ROW
 #undef NERF_WG16_FN
   int n = 1;
-  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess) {
+    (void)hipGetLastError();  // not left for the launch checks that follow
+    n = 1;
+  }
+  if (n < 1) n = 1;
   v = n < 3 ? n : 3;
   memo[i][j].store(v, std::memory_order_relaxed);
   return v;
