@@ -212,10 +212,13 @@ int32_t nerf_mlp_backward(const nerf_mlp_desc* desc, int32_t precision, const vo
  * (accumulate != 0) sum over the M samples of X^T dz for every layer, and the
  * bias sums.  stash and dz hold M samples as the forward and backward wrote
  * them; a sub-range of a larger batch starts and ends on multiples of
- * nerf_mlp_sample_block() (or at the batch's end).  workspace: a device buffer of workspace_bytes bytes, at least
- * nerf_mlp_wgrad_workspace_bytes(desc, M) (a bound for both precisions); a
- * smaller buffer than the call's launches use is rejected with
- * NERF_ERR_INVALID_ARG before anything is launched.
+ * nerf_mlp_sample_block() (or at the batch's end).  workspace: a device buffer of
+ * workspace_bytes bytes: nerf_mlp_wgrad_workspace_bytes(desc, M) (a bound for
+ * both precisions) lets a bf16 call keep every layer group's split-K partial
+ * tiles side by side and sum them in one launch; a smaller one down to what one
+ * group uses works too (each group summed before the next, the same result);
+ * a buffer smaller than that is rejected with NERF_ERR_INVALID_ARG before
+ * anything is launched.
  * The summation order is fixed (deterministic, no float atomics).
  * M == 0 (an empty batch): grad_flat is zeroed (or left as is when
  * accumulating); stash, dz and workspace may then be NULL.  Every other entry
@@ -238,7 +241,8 @@ int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int6
  * samples in either precision).  Every call on one GPU shares that GPU's array:
  * calls that overlap on one GPU (other streams or threads) race.  Nothing in the
  * array needs clearing (each call writes every partial it reads), and the result
- * is bitwise that of nerf_mlp_weight_grads (same launches, same order). */
+ * is bitwise that of nerf_mlp_weight_grads (the same partial tiles, summed in the
+ * same order). */
 #define NERF_WGRAD_STATIC_SCRATCH_BYTES (256LL * (256 * 256 + 256) * 4) /* 64.25 MiB */
 int32_t nerf_mlp_weight_grads_noalloc(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
                                       const void* stash, const void* dz, float* grad_flat,

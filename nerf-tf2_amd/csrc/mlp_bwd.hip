@@ -967,11 +967,11 @@ __device__ __forceinline__ void reduce_job(const ReduceArgs& a, int64_t blk, int
 
 __global__ void wgrad_reduce_kernel(ReduceArgs a) { reduce_job(a, blockIdx.x, gridDim.x); }
 
-// The reductions of a group of jobs in one launch: job j owns blocks
-// [first[j], first[j + 1]).
+// The reductions of several jobs (a group, or every job of the call) in one
+// launch: job j owns blocks [first[j], first[j + 1]).
 struct ReduceBatch {
-  ReduceArgs job[kMaxJobs];
-  int first[kMaxJobs + 1];
+  ReduceArgs job[2 * NERF_MAX_LAYERS];
+  int first[2 * NERF_MAX_LAYERS + 1];
   int njobs;
 };
 
@@ -1138,8 +1138,11 @@ static int64_t wgrad_workspace_floats(const MlpPlan& P, int64_t M) {
 }
 
 // The bf16 launches: jobs grouped by tile shape (up to kMaxJobs per launch),
-// each group with sg splits per job; a group's partial tiles start at the
-// workspace base (groups run one after another on the stream).
+// each group with sg splits per job.  A group's partial tiles start at the
+// workspace base and are summed before the next group's GEMM overwrites them,
+// or, where the workspace holds every group's tiles side by side, all groups'
+// GEMMs run first and one launch sums every split (the same sums, fewer
+// launches).
 struct Wg16Group {
   int members[kMaxJobs];
   int n, kb, nb;
@@ -1155,7 +1158,10 @@ static int64_t wg16_group_floats(const Wg16Group& g, const WgradJob* jobs) {
   return f;
 }
 
-static int wg16_groups(const MlpPlan& P, int64_t M, const WgradJob* jobs, int nj, Wg16Group* groups) {
+// bound: assume the most workgroups per CU any tile shape gets (3) instead of
+// asking the runtime, for a size that needs no device.
+static int wg16_groups(const MlpPlan& P, int64_t M, const WgradJob* jobs, int nj, Wg16Group* groups,
+                       bool bound = false) {
   bool done[2 * NERF_MAX_LAYERS] = {};
   const int64_t row_bytes = 2 * (P.stash_per_sample > P.dz_per_sample ? P.stash_per_sample : P.dz_per_sample);
   int ng = 0;
@@ -1192,7 +1198,8 @@ static int wg16_groups(const MlpPlan& P, int64_t M, const WgradJob* jobs, int nj
     int64_t per = 0;  // floats per split
     for (int k = 0; k < g.n; ++k) per += jobs[g.members[k]].K * (int64_t)jobs[g.members[k]].N + jobs[g.members[k]].N;
     if (per < 16384) continue;
-    int64_t sg = (256 * wg16_per_cu(g.kb, g.nb) + g.n - 1) / g.n;
+    const int pc = bound ? 3 : wg16_per_cu(g.kb, g.nb);
+    int64_t sg = (256 * pc + g.n - 1) / g.n;
     if (sg > cap) sg = cap;
     if (sg > most / per) sg = most / per;
     if (sg > g.sg) g.sg = sg;
@@ -1220,17 +1227,39 @@ static int64_t wgrad_used_floats(const MlpPlan& P, int32_t precision, int64_t M)
   return splits_for(M) * (kmax * P.W + P.W);
 }
 
+// Floats of every bf16 group's partial tiles side by side (see Wg16Group).
+static int64_t wg16_stacked_floats(const MlpPlan& P, int64_t M, bool bound) {
+  WgradJob jobs[2 * NERF_MAX_LAYERS];
+  const int nj = wgrad_jobs(P, jobs);
+  Wg16Group groups[2 * NERF_MAX_LAYERS];
+  const int ng = wg16_groups(P, M, jobs, nj, groups, bound);
+  int64_t f = 0;
+  for (int i = 0; i < ng; ++i) f += wg16_group_floats(groups[i], jobs);
+  return f;
+}
+
+// The workspace callers are told to allocate: room for either precision, in
+// bf16 for every group's partial tiles side by side.
+static int64_t wgrad_workspace_bytes_for(const nerf_mlp_desc* desc, int64_t M) {
+  MlpPlan P, P16;
+  if (nerf_plan_from_desc(desc, NERF_PREC_F32, &P) || M < 0) return -1;
+  int64_t f = wgrad_workspace_floats(P, M);
+  if (nerf_plan_from_desc(desc, NERF_PREC_BF16, &P16) == 0) {
+    const int64_t st = wg16_stacked_floats(P16, M, true);
+    if (st > f) f = st;
+  }
+  return f * 4 + 256;
+}
+
 extern "C" int64_t nerf_mlp_wgrad_workspace_bytes(const nerf_mlp_desc* desc, int64_t M) {
-  MlpPlan P;
-  if (nerf_plan_from_desc(desc, 0, &P) || M < 0) return -1;
-  return wgrad_workspace_floats(P, M) * 4 + 256;
+  return wgrad_workspace_bytes_for(desc, M);
 }
 
 // The launches of one weight-gradient call (M > 0) into the partial-tile buffer
-// `part` of at least wgrad_used_floats(P, precision, M) floats.
+// `part` of part_floats >= wgrad_used_floats(P, precision, M) floats.
 static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const void* stash,
                          const void* dz, float* grad_flat, int32_t accumulate, float* part,
-                         hipStream_t s) {
+                         int64_t part_floats, hipStream_t s) {
   WgradJob jobs[2 * NERF_MAX_LAYERS];
   const int nj = wgrad_jobs(P, jobs);
   const int64_t splits = splits_for(M);
@@ -1261,6 +1290,13 @@ static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const v
     Wg16Group groups[2 * NERF_MAX_LAYERS];
     const int ng = wg16_groups(P, M, jobs, nj, groups);
     NERF_CHECK_ARG(ng > 0, "nerf_mlp_weight_grads: layer too wide for the bf16 weight GEMM");
+    int64_t stacked = 0;
+    for (int gi = 0; gi < ng; ++gi) stacked += wg16_group_floats(groups[gi], jobs);
+    const bool stack = stacked <= part_floats;  // every group's tiles side by side
+    ReduceBatch all;
+    all.njobs = 0;
+    all.first[0] = 0;
+    float* base = part;
     for (int gi = 0; gi < ng; ++gi) {
       const Wg16Group& G = groups[gi];
       const int n = G.n;
@@ -1276,7 +1312,7 @@ static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const v
       b.splits = (int)sg;
       b.sb_shift = P.sblk == 32 ? 5 : 0;
       b.njobs = n;
-      float* wp = part;
+      float* wp = base;
       for (int i = 0; i < n; ++i) {
         const WgradJob& J = jobs[G.members[i]];
         b.job[i].xoff = J.xoff;
@@ -1290,15 +1326,21 @@ static int32_t wgrad_run(const MlpPlan& P, int32_t precision, int64_t M, const v
       }
       NERF_CHECK_ARG(launch_wgrad_bf16(b, G.kb, G.nb, s) == 0,
                      "nerf_mlp_weight_grads: no bf16 weight GEMM for this tile shape and stash layout");
-      ReduceBatch rbt;
-      rbt.njobs = n;
-      rbt.first[0] = 0;
-      for (int i = 0; i < n; ++i) {
-        rbt.job[i] = reduce_args(jobs[G.members[i]], b.job[i].part, b.job[i].bpart, (int)sg);
-        rbt.first[i + 1] = rbt.first[i] + reduce_blocks(jobs[G.members[i]]);
+      if (stack) base = wp;
+      ReduceBatch one;
+      ReduceBatch& rbt = stack ? all : one;
+      if (!stack) {
+        rbt.njobs = 0;
+        rbt.first[0] = 0;
       }
-      wgrad_reduce_batch_kernel<<<(unsigned)rbt.first[n], 256, 0, s>>>(rbt);
+      for (int i = 0; i < n; ++i) {
+        const int k = rbt.njobs++;
+        rbt.job[k] = reduce_args(jobs[G.members[i]], b.job[i].part, b.job[i].bpart, (int)sg);
+        rbt.first[k + 1] = rbt.first[k] + reduce_blocks(jobs[G.members[i]]);
+      }
+      if (!stack) wgrad_reduce_batch_kernel<<<(unsigned)rbt.first[rbt.njobs], 256, 0, s>>>(rbt);
     }
+    if (stack) wgrad_reduce_batch_kernel<<<(unsigned)all.first[all.njobs], 256, 0, s>>>(all);
   } else {
     for (int t = 0; t < nj; ++t) {
       const WgradJob& J = jobs[t];
@@ -1389,7 +1431,8 @@ extern "C" int32_t nerf_mlp_weight_grads_noalloc(const nerf_mlp_desc* desc, int3
     nerf_set_error("nerf_mlp_weight_grads_noalloc: no device address for the static scratch");
     return NERF_ERR_LAUNCH;
   }
-  return wgrad_run(P, precision, M, stash, dz, grad_flat, accumulate, part, s);
+  return wgrad_run(P, precision, M, stash, dz, grad_flat, accumulate, part,
+                   (int64_t)NERF_WGRAD_STATIC_SCRATCH_BYTES / 4, s);
 }
 
 extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t precision, int64_t M,
@@ -1409,7 +1452,7 @@ extern "C" int32_t nerf_mlp_weight_grads(const nerf_mlp_desc* desc, int32_t prec
                  "nerf_mlp_weight_grads: workspace of %lld bytes, this call needs %lld "
                  "(nerf_mlp_wgrad_workspace_bytes gives %lld for M = %lld)",
                  (long long)workspace_bytes, (long long)(need * 4),
-                 (long long)(wgrad_workspace_floats(P, M) * 4 + 256), (long long)M);
+                 (long long)wgrad_workspace_bytes_for(desc, M), (long long)M);
   return wgrad_run(P, precision, M, stash, dz, grad_flat, accumulate,
-                   reinterpret_cast<float*>(workspace), s);
+                   reinterpret_cast<float*>(workspace), workspace_bytes / 4, s);
 }
