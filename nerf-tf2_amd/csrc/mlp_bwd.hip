@@ -391,8 +391,9 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
 // the split's last sample, so samples past it read as zero.  Tile split over
 // the 4 waves: with NB >= 4 column blocks, wave w owns column blocks w, w + 4
 // and every row block; otherwise wave w owns row blocks w, w + 4.  Rows k >= K
-// and columns n >= N of the tile hold whatever the neighbouring features are
-// and are not written out.  Bias sums (sum over samples of dz) are taken from
+// and columns n >= N of the tile are not written out (they hold the
+// neighbouring features up to the next 8-feature chunk, zeros past it: the
+// generic kernel does not load whole chunks beyond K or N).  Bias sums (sum over samples of dz) are taken from
 // the staged bf16 dz.
 // ---------------------------------------------------------------------------
 constexpr int kWC = 64;       // samples per chunk
@@ -560,24 +561,28 @@ __global__ __launch_bounds__(256, wg16_min_waves(KB, NB)) void wgrad_bf16_kernel
   const uint32_t bzf = (uint32_t)((((J.zoff >> 3) + chz) * 32 + riz) * 16);
   const uint32_t chx32 = (uint32_t)((J.xoff >> 3) + chx), chz32 = (uint32_t)((J.zoff >> 3) + chz);
   const uint32_t ldx32 = (uint32_t)a.ldx, ldz32 = (uint32_t)a.ldz, rows32 = (uint32_t)rows;
+  // chunks past the job's K (N) features: offsets outside the buffer, so that the
+  // padding of a narrow tile reads zeros instead of the neighbouring features
+  const uint32_t xoob = chx < ((J.K + 7) >> 3) ? 0u : 0x80000000u;
+  const uint32_t zoob = chz < ((J.N + 7) >> 3) ? 0u : 0x80000000u;
   auto load_chunk = [&](auto set, int c) {
     constexpr int S = decltype(set)::value;
     if ((int64_t)(c + 1) * kWC <= (rows & ~(int64_t)31)) {  // whole blocks (wave-uniform)
 #pragma unroll
       for (int k = 0; k < CWX; ++k)
-        xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx), 0, kNT);
+        xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rx, blk_fast_off<RIX>(bxf, c, k, a.ldx) | xoob, 0, kNT);
 #pragma unroll
       for (int k = 0; k < CWZ; ++k)
-        zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz), 0, kNT);
+        zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(rz, blk_fast_off<RIZ>(bzf, c, k, a.ldz) | zoob, 0, kNT);
     } else {
 #pragma unroll
       for (int k = 0; k < CWX; ++k)
         xr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(
-            rx, blk32_chunk_off((uint32_t)(c * kWC + k * RIX + rix), chx32, ldx32, rows32), 0, kNT);
+            rx, blk32_chunk_off((uint32_t)(c * kWC + k * RIX + rix), chx32, ldx32, rows32) | xoob, 0, kNT);
 #pragma unroll
       for (int k = 0; k < CWZ; ++k)
         zr[S][k] = __builtin_amdgcn_raw_buffer_load_b128(
-            rz, blk32_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), chz32, ldz32, rows32), 0, kNT);
+            rz, blk32_chunk_off((uint32_t)(c * kWC + k * RIZ + riz), chz32, ldz32, rows32) | zoob, 0, kNT);
     }
   };
   auto store_chunk = [&](auto set, int buf) {
