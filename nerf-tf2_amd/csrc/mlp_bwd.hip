@@ -382,19 +382,20 @@ __global__ __launch_bounds__(256) void wgrad_gemm_kernel(GemmArgs a) {
 // bf16 weight-gradient GEMM, one pass.  Workgroup s owns the sample range of
 // split s and computes the whole K x N (<= 256 x 256) partial tile, so every X
 // (bf16 stash) and dz (bf16) row is read once per layer.  64-sample chunks are
-// staged transposed, [feature][sample], into a double-buffered LDS tile as
-// packed sample pairs, so each lane's MFMA operand (8 consecutive samples of one
-// feature) is one ds_read_b128; v_mfma_f32_32x32x16_bf16, fp32 accumulation.
-// While chunk c is multiplied, chunk c+1 goes from registers into the other
-// buffer and chunk c+2 is loaded into registers; one barrier per chunk and no
-// branch in the loop.  Loads go through buffer resources whose range ends at
-// the split's last sample, so samples past it read as zero.  Tile split over
-// the 4 waves: with NB >= 4 column blocks, wave w owns column blocks w, w + 4
-// and every row block; otherwise wave w owns row blocks w, w + 4.  Rows k >= K
-// and columns n >= N of the tile are not written out (they hold the
-// neighbouring features up to the next 8-feature chunk, zeros past it: the
-// generic kernel does not load whole chunks beyond K or N).  Bias sums (sum over samples of dz) are taken from
-// the staged bf16 dz.
+// staged transposed, [feature][sample], into an LDS tile as packed sample
+// pairs, so each lane's MFMA operand (8 consecutive samples of one feature) is
+// one ds_read_b128; v_mfma_f32_32x32x16_bf16, fp32 accumulation.  While chunk c
+// is multiplied, chunk c+1 goes from registers into the other LDS buffer (or,
+// with one buffer, into the same one after a barrier) and chunk c+2 (and c+3,
+// with two register sets) is in flight; no branch in the loop.  Loads go through
+// buffer resources whose range ends at the split's last sample, so samples past
+// it read as zero.  Tile split over the 4 waves of wgrad_bf16_kernel: with
+// NB >= 4 column blocks, wave w owns column blocks w, w + 4 and every row
+// block; otherwise wave w owns row blocks w, w + 4.  Rows k >= K and columns
+// n >= N of the tile are not written out (they hold the neighbouring features
+// up to the next 8-feature chunk, zeros past it: the generic kernel does not
+// load whole chunks beyond K or N).  Bias sums (sum over samples of dz) are
+// taken from the staged bf16 dz.
 // ---------------------------------------------------------------------------
 constexpr int kWC = 64;       // samples per chunk
 constexpr int kWP = kWC + 8;  // bf16 per LDS row (16-byte aligned rows, staggered banks)
