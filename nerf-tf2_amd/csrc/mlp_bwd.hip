@@ -2,14 +2,15 @@
 // takes through model.py:355-392) and its weight gradients.
 //
 // nerf_mlp_backward: one wave per 32 samples walks the layers top-down (the
-// reference topology in fp32 runs the 16-sample register-resident twin of
-// mlp_bwd_f32r.h, same rows in and out).
+// reference topology runs the register-resident twin of mlp_bwd_f32r.h in fp32
+// and the eight-wave kernel of mlp_bwd_bf16ws8.h in bf16, same rows in and out).
 //   rgb = sigmoid(z):  dz_rgb = d_rgb * rgb (1 - rgb)
 //   sigma = relu(z):   dz_sigma = d_sigma * [sigma > 0]      (softplus: * (1 - e^-sigma))
 //   dh = W dz (MFMA with the transposed packed weights; the accumulators of one
 //   layer are the B operand of the next, as in the forward), dz = dh * [h > 0]
-//   using the forward's stashed activations for the ReLU masks.
-//   Every pre-activation gradient dz_l is written row-major for the weight GEMMs.
+//   using the forward's stashed activations (or its ReLU bits) for the masks.
+//   Every pre-activation gradient dz_l is written for the weight GEMMs: fp32
+//   rows, or bf16 blocked by 32 samples (MlpPlan::sblk).
 // nerf_mlp_weight_grads: dW_l = X_l^T dz_l and db_l = sum dz_l over the samples,
 //   as a split-K MFMA GEMM whose partial tiles are summed in a fixed order.
 #include <atomic>
