@@ -83,3 +83,50 @@ def test_bf16_weight_grads_match_fp32_kernels(nerf, net, M):
             e = float((a - b).norm()) / scale
             assert e < 1e-4, (name, what, e)
     assert off == gb.numel()
+
+
+@pytest.mark.parametrize("net", ["nerf", "proposal"])
+@pytest.mark.parametrize("M", [4099, 70000])
+def test_bf16_weight_grads_workspace_modes(nerf, net, M):
+    """A workspace of nerf_mlp_wgrad_workspace_bytes holds every layer group's
+    split-K partial tiles side by side (one launch sums them all); one of exactly
+    the size the call needs (what an undersized call's error names) makes each
+    group's tiles summed before the next group's GEMM.  Both sum the same partials
+    in the same order: the gradients agree bit for bit, fresh and accumulated."""
+    L = nerf._lib
+    cfg = nerf.MLPConfig() if net == "nerf" else nerf.proposal.PROPOSAL_CONFIG
+    bf = nerf.NerfMLP("fine", cfg, "bf16", seed=5)
+    d = ctypes.byref(bf._desc)
+    g = torch.Generator().manual_seed(3 * M + len(net))
+    xyz = (torch.rand((M, 3), generator=g) * 6 - 3).float().to(DEV)
+    dirs = torch.nn.functional.normalize(torch.randn((M, 3), generator=g), dim=1).float().to(DEV)
+    g_sig = torch.randn((M, 1), generator=g).float().to(DEV)
+    g_rgb = torch.randn((M, 3), generator=g).float().to(DEV) if cfg.l_dir else None
+    rgb = torch.empty((M, 3), device=DEV) if cfg.l_dir else None
+    sigma = torch.empty((M, 1), device=DEV)
+    stash = torch.zeros((M, bf.stash_per_sample), dtype=torch.bfloat16, device=DEV)
+    dz = torch.zeros((M, bf.dz_per_sample), dtype=torch.bfloat16, device=DEV)
+    bf._launch_forward(M, xyz, dirs, None, None, None, rgb, sigma, stash)
+    L.call("nerf_mlp_backward", d, bf.prec, L.ptr(bf.packed), M, L.ptr(g_rgb), L.ptr(g_sig), L.ptr(rgb),
+           L.ptr(sigma), L.ptr(stash), L.ptr(dz), L.stream())
+    lib = L.load()
+    full = int(lib.nerf_mlp_wgrad_workspace_bytes(d, M))
+    fake = ctypes.c_void_p(0x1000)  # rejected before any launch: never dereferenced
+    rc = lib.nerf_mlp_weight_grads(d, bf.prec, M, L.ptr(stash), L.ptr(dz), fake, 0, fake, 0, None)
+    assert rc == L.NERF_ERR_INVALID_ARG
+    need = int(lib.nerf_last_error().decode().split("needs ")[1].split()[0])
+    assert 0 < need <= full
+    init = torch.randn((bf.n_params,), generator=g).float().to(DEV)
+    outs = []
+    for nbytes in (full, need):
+        ws = torch.empty((nbytes,), dtype=torch.uint8, device=DEV)
+        fresh = torch.empty((bf.n_params,), device=DEV)
+        acc = init.clone()
+        for out, accumulate in ((fresh, 0), (acc, 1)):
+            L.call("nerf_mlp_weight_grads", d, bf.prec, M, L.ptr(stash), L.ptr(dz), L.ptr(out), accumulate,
+                   L.ptr(ws), nbytes, L.stream())
+        torch.cuda.synchronize()
+        outs.append((fresh.cpu(), acc.cpu()))
+    assert bool(torch.isfinite(outs[0][0]).all()) and float(outs[0][0].norm()) > 0
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
