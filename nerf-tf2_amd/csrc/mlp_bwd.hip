@@ -1189,11 +1189,12 @@ static int wg16_groups(const MlpPlan& P, int64_t M, const WgradJob* jobs, int nj
   }
   // Where the generic kernel fits several workgroups per CU, a group gets that
   // many times the splits, as far as its partial tiles stay within the largest
-  // group's (the workspace a call uses does not grow) and its tiles are large
-  // enough (16,384 floats per split) for the split sums to run wide: the sums
-  // over the splits of a narrow tile (one thread per output) cost more than
-  // the extra workgroups gain.
-  int64_t most = 0;
+  // group's or the allocation-free entry point's static scratch, whichever is
+  // larger (so that entry point takes every batch it took before), and its tiles
+  // are large enough (16,384 floats per split) for the split sums to run wide:
+  // the sums over the splits of a narrow tile (one thread per output) cost more
+  // than the extra workgroups gain.
+  int64_t most = (int64_t)NERF_WGRAD_STATIC_SCRATCH_BYTES / 4;
   for (int i = 0; i < ng; ++i) {
     const int64_t f = wg16_group_floats(groups[i], jobs);
     if (f > most) most = f;
