@@ -35,10 +35,23 @@ __global__ __launch_bounds__(256) void interlevel_kernel(
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const float fr = far[ray];
+  // this lane's fine intervals i = lane + 64 it, loaded together up front
+  constexpr int kIt = kMaxSf / 64;
+  float lo_v[kIt], hi_v[kIt], w_v[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int i = lane + 64 * it;
+    lo_v[it] = i < Sf ? tf[ray * Sf + i] : 0.f;
+    hi_v[it] = i + 1 < Sf ? tf[ray * Sf + i + 1] : fr;
+    w_v[it] = i < Sf ? wf[ray * Sf + i] : 0.f;
+  }
   float ms = 0.f, ls = 0.f;
-  for (int i = lane; i < Sf; i += 64) {
-    const float lo_t = tf[ray * Sf + i];
-    const float hi_t = i + 1 < Sf ? tf[ray * Sf + i + 1] : fr;
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int i = lane + 64 * it;
+    if (i >= Sf) continue;
+    const float lo_t = lo_v[it];
+    const float hi_t = hi_v[it];
     // k_lo: last edge <= lo_t ; k_hi: first edge >= hi_t
     int a = 0, b = Sp + 1;
     while (a < b) {
@@ -58,7 +71,7 @@ __global__ __launch_bounds__(256) void interlevel_kernel(
     khi = khi + dilate > Sp ? Sp : khi + dilate;
     float bound = 0.f;
     for (int k = klo; k < khi; ++k) bound += s_w[w][k];
-    const float wi = wf[ray * Sf + i];
+    const float wi = w_v[it];
     const float mi = fmaxf(wi - bound, 0.f);
     ms += mi;
     ls += mi * mi / (wi + 1e-7f);
@@ -78,6 +91,7 @@ __global__ __launch_bounds__(256) void interlevel_kernel(
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (lane < Sp) {
     float g = 0.f;
+#pragma unroll 8
     for (int i = 0; i < Sf; ++i)
       if (s_lo[w][i] <= lane && lane < s_hi[w][i]) g += s_coef[w][i];
     g_wp[ray * Sp + lane] = g * gscale;
